@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""MG-PCG benchmark: DOF-iterations/s and time-to-solution of the equilibrium
+solve K(rho) u = f on the 192x96x96 cantilever (5.4M DOFs, 5-level GMG),
+FP64 preconditioner vs mixed (FP32 V-cycle inside FP64 PCG), on one MI355X.
+
+One step = one full MG-PCG solve from x0 = 0 to ||r||/||f|| <= 1e-8 with the
+hierarchy, densities and load already resident in HBM (setup is timed and
+reported separately, as the reference's setupK / setupPrec / system split).
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "topopt-slp-3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_VEC_PEAK_TFLOPS = 78.6
+FP32_VEC_PEAK_TFLOPS = 78.6   # packed FP32 is disabled in this build; unpacked FP32 = FP64 vector rate
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", default="192x96x96")
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def cpu_baseline():
+    """FP64 CPU restatement of the reference's assembled GMG-PCG path (the
+    oracle) timed on this host: solve phase of the 96x48x48 cantilever."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle
+    from tmg import problem as P
+    cores = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    prob = P.cantilever(96, 48, 48)
+    x = P.random_density(96, 48, 48, 2)
+    o = oracle.Oracle(prob)
+    t0 = time.perf_counter()
+    o.assemble(x)
+    o.setup_gmg(4, 0.6)
+    t_setup = time.perf_counter() - t0
+    times, its = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        u, it, rr = o.solve(prob.f)
+        times.append(time.perf_counter() - t0)
+        its.append(it)
+    t = statistics.median(times)
+    it = its[0]
+    return {
+        "value": prob.nfree * (it - 1) / t,
+        "unit": "DOF-iters/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "kind": "port",
+        "sample": (f"FP64 C/OpenMP restatement of the reference's assembled SolveMG path on the 96x48x48 "
+                   f"cantilever ({prob.nfree} free DOFs, filtered random densities seed 2, 4-level V(1,1) "
+                   f"Jacobi omega 0.6, tol 1e-8); solve phase only, median of 3: {t:.3f} s, {it - 1} PCG "
+                   f"iterations; setup (assembly + Galerkin + Cholesky) {t_setup:.2f} s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import tmg
+    from tmg import problem as P
+
+    nelx, nely, nelz = (int(v) for v in args.grid.split("x"))
+    L = args.levels
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    log(f"problem {nelx}x{nely}x{nelz}, {L} levels, seed {args.seed}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, args.seed)
+    f_full = torch.from_numpy(prob.full(prob.f)).to(dev)
+    golden = {}
+    gpath = os.path.join(ROOT, "tests", "golden", f"oracle_{nelx}x{nely}x{nelz}_seed{args.seed}.json")
+    if os.path.exists(gpath):
+        golden = json.load(open(gpath))
+
+    results = {}
+    solvers = {}
+    for prec in ("fp64", "mixed"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = tmg.Solver(prob, L, precision=prec)
+        t1 = time.perf_counter()
+        s.update_density(x)
+        t2 = time.perf_counter()
+        solvers[prec] = s
+        xd = torch.zeros(prob.ndof, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            xd.zero_()
+            s.solve_device(xd, f_full)
+        torch.cuda.synchronize()
+        times, its = [], []
+        for _ in range(args.steps):
+            xd.zero_()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            it, rr, rc = s.solve_device(xd, f_full)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+            its.append(it)
+        u = xd.cpu().numpy()[prob.freedofs - 1]
+        # true residual with the FP64 operator of the FP64 hierarchy
+        Ku = solvers["fp64"].apply(1, u)
+        relres_true = float(np.linalg.norm(prob.f - Ku) / np.linalg.norm(prob.f))
+        C = float(prob.f @ u)
+        t_solve = sum(times) / len(times)
+        results[prec] = dict(it=its[-1], relres=rr, relres_true=relres_true, compliance=C, t_solve=t_solve,
+                             t_setup_structure=t1 - t0, t_setup_operator=t2 - t1,
+                             dof_iters=prob.nfree * (its[-1] - 1) / t_solve)
+        log(f"{prec}: it={its[-1]} t_solve={t_solve * 1e3:.2f} ms relres={rr:.2e} true={relres_true:.2e} C={C:.10e} "
+            f"setup={t1 - t0:.3f}+{t2 - t1:.3f} s DOF-iters/s={results[prec]['dof_iters']:.3e}")
+
+    for prec in results:
+        if "compliance" in golden:
+            results[prec]["compliance_rel_err_vs_oracle"] = abs(results[prec]["compliance"] - golden["compliance"]) / abs(golden["compliance"])
+    results["mixed"]["compliance_rel_err_vs_fp64"] = abs(results["mixed"]["compliance"] - results["fp64"]["compliance"]) / abs(results["fp64"]["compliance"])
+
+    # dominant kernels, HIP events on the solver's stream
+    n_dof, n_el = prob.ndof, prob.nelem
+    sm = solvers["mixed"]
+    sf = solvers["fp64"]
+    reps = args.kernel_reps
+    k = {
+        "fine_apply_fp64_ms": sf.time_kernel(0, reps),
+        "fine_jacobi_fp32_ms": sm.time_kernel(1, reps),
+        "fine_jacobi_fp64_ms": sf.time_kernel(1, reps),
+        "l2_jacobi_fp32_ms": sm.time_kernel(2, reps) if L >= 3 else None,
+        "l2_jacobi_fp64_ms": sf.time_kernel(2, reps) if L >= 3 else None,
+        "vcycle_mixed_ms": sm.time_kernel(3, max(reps // 5, 3)),
+        "vcycle_fp64_ms": sf.time_kernel(3, max(reps // 5, 3)),
+    }
+    log("kernels:", json.dumps(k))
+    # fine-level Jacobi sweep in FP32: read x, b, invD; write x; read E
+    bytes_jac32 = 4 * 4 * n_dof + 4 * n_el
+    flops_apply = 1152 * n_el
+    t = k["fine_jacobi_fp32_ms"] * 1e-3
+    achieved = bytes_jac32 / t / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": "k_ebe<float,JACOBI> (fine-level fused apply + damped Jacobi, FP32)",
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "bytes_per_launch": bytes_jac32, "avg_launch_ms": round(k["fine_jacobi_fp32_ms"], 5),
+        "valu_tflops": round(flops_apply / t / 1e12, 2), "valu_peak_tflops": FP32_VEC_PEAK_TFLOPS,
+        "valu_frac": round(flops_apply / t / 1e12 / FP32_VEC_PEAK_TFLOPS, 4),
+    }
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline()
+        log("cpu:", json.dumps(cpu))
+    mixed, fp64 = results["mixed"], results["fp64"]
+    line = {
+        "metric": "MG-PCG DOF-iters/s + solve time @5.4M DOFs (FP64 vs mixed); served p99",
+        "value": round(mixed["dof_iters"], 1),
+        "unit": "DOF-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["t_solve"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (PCG, outer apply, coarse) + f32 (V-cycle)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cantilever {nelx}x{nely}x{nelz} Q1, {prob.nfree} free DOFs, {L}-level GMG V(1,1) "
+                        f"Jacobi omega 0.6, filtered random densities seed {args.seed}, tol 1e-8",
+            "n_free_dofs": prob.nfree,
+            "fp64": {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in fp64.items()},
+            "mixed": {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in mixed.items()},
+            "speedup_mixed_vs_fp64_time_to_solution": round(fp64["t_solve"] / mixed["t_solve"], 3),
+            "kernels_ms": k,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * tmg.h -- C-ABI of the MI355X multigrid equilibrium solver (libtmg.so).
+ *
+ * The solver replaces the equilibrium solve of TopOpt-SLP-3D: the geometric
+ * multigrid preconditioned CG for K(rho) u = f on a structured grid of Q1
+ * hexahedra, with the multigrid hierarchy resident in device memory.
+ *
+ * Each entry point below replaces one MATLAB function of the reference (cited
+ * per declaration). Plain C types only: host pointers are caller-owned,
+ * vectors are double, free-DOF vectors are compact in str.freedofs order (the
+ * reference's u(str.freedofs)), indices are 1-based where the reference's are.
+ * Device memory is owned by the handle. Calls on one handle are serialised on
+ * the handle's own HIP stream; different handles are independent.
+ *
+ * Return convention: 0 = ok, > 0 = solve did not reach the tolerance
+ * (TMG_NOT_CONVERGED), < 0 = error; tmg_last_error() gives the message.
+ */
+#ifndef TMG_H
+#define TMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tmg_solver tmg_solver;   /* device-resident GMG hierarchy        */
+typedef struct tmg_filter tmg_filter;   /* device-resident density filter       */
+
+/* Precision of the preconditioner. The PCG recurrences, dot products, the
+ * outer operator apply q = K d, the coarsest-grid factorization and the
+ * gradient are FP64 in every mode. */
+enum {
+    TMG_FP64 = 0,   /* V-cycle in FP64                                          */
+    TMG_MIXED = 1   /* V-cycle (smoother, transfers, level operators) in FP32   */
+};
+
+enum {
+    TMG_OK = 0,
+    TMG_NOT_CONVERGED = 1,
+    TMG_ERR_ARG = -1,
+    TMG_ERR_UNSUPPORTED = -2,
+    TMG_ERR_OOM = -3,
+    TMG_ERR_NOT_SPD = -4,
+    TMG_ERR_HIP = -5,
+    TMG_ERR_STATE = -6,
+    TMG_ERR_BUDGET = -7
+};
+
+/* The fields of the reference's `str` struct that the solve reads
+ * (Implementation/TopOpt3D.m:4-43, Implementation/EnterData.m:79-86,147-156). */
+typedef struct tmg_problem {
+    int nelx, nely, nelz;       /* str.nelx, str.nely, str.nelz                  */
+    double el, eh, ew;          /* str.el, str.eh, str.ew                         */
+    double E0;                  /* str.E                                          */
+    double nu;                  /* str.nu                                         */
+    double penal;               /* p (SIMP penalty)                               */
+    double emin;                /* emin (Young's modulus of void)                 */
+    const int64_t *freedofs;    /* str.freedofs, 1-based, strictly increasing     */
+    int64_t nfree;
+} tmg_problem;
+
+/* The `mg` struct (Implementation/TopOpt3D.m:171-188). Only cycle = 0
+ * (V-cycle), smoother = 0 (Jacobi) and opchol = 0 (coarse Cholesky) are
+ * accepted, the configuration of the reference's default GMG path. */
+typedef struct tmg_mg_params {
+    int ngrids;          /* mg.ngrids                                             */
+    int cycle;           /* mg.cycle        (0 = V)                               */
+    int smoother;        /* mg.smoother     (0 = Jacobi)                          */
+    double omega;        /* mg.omega                                              */
+    int smoothiter1;     /* mg.smoothiter1                                        */
+    int smoothiter2;     /* mg.smoothiter2                                        */
+    double tolMG;        /* mg.tolMG                                              */
+    int maxiterMG;       /* mg.maxiterMG                                          */
+    int opchol;          /* mg.opchol       (0 = Cholesky on the coarsest grid)   */
+    int nmaxchol;        /* mg.nmaxchol                                           */
+} tmg_mg_params;
+
+/* Level structure, prolongations, free-DOF sets per level and device memory
+ * for the hierarchy. Replaces the structural part of
+ * [A,P,M,L,U,R,perm] = SetupGMG(K,str,mg,elem) (Implementation/SetupGMG.m:16-44).
+ * The operators are built by tmg_update_density. mg->ngrids must already obey
+ * the level rules of Implementation/StartStrSLP.m:107-133. */
+int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, tmg_solver **out);
+
+/* Operator refresh for new filtered densities x (n_el, element order):
+ * K = sum_e (emin + (E0-emin) x_e^p) Ke, Galerkin A{i} = P'A{i-1}P, Jacobi
+ * M{i} = (1/omega) diag(A{i}), coarse Cholesky. Replaces
+ * GlobalStiffMatrix (Implementation/GlobalStiffMatrix.m:16-18,64-65) plus the
+ * numeric part of SetupGMG (Implementation/SetupGMG.m:45-75) and the refresh
+ * loop of Implementation/StrSLP.m:319,345-365. Returns TMG_ERR_NOT_SPD when
+ * the coarse factorization fails. */
+int tmg_update_density(tmg_solver *h, const double *x);
+/* Same with x already in device memory. */
+int tmg_update_density_device(tmg_solver *h, const double *d_x);
+
+/* [x,it] = SolveMG(x0,f,Ac,P,M,L,U,R,perm,ngrids,cycle,smoother,
+ *                  smoothiter1,smoothiter2,tolMG,maxiterMG,opchol,opcond,pcgp)
+ * (Implementation/SolveMG.m:1-55). x0, f, x_out: nfree host vectors.
+ * *it gets the reference's count (PCG iterations + 1); *relres the final
+ * recursive ||r||/||f||. Returns TMG_NOT_CONVERGED when maxiterMG was hit. */
+int tmg_solve(tmg_solver *h, const double *x0, const double *f, double *x_out, int *it, double *relres);
+/* Same on device-resident full-length vectors (3*nnodes, fixed DOFs zero):
+ * d_x holds x0 on entry and the solution on exit. Blocks until done. */
+int tmg_solve_device(tmg_solver *h, double *d_x, const double *d_f, int *it, double *relres);
+
+/* x = VCycle(A,b,zeros,niter1,niter2,ngrids,1,...) (Implementation/VCycle.m:1-45),
+ * one V-cycle at the finest level on nfree host vectors. */
+int tmg_vcycle(tmg_solver *h, const double *b, double *x);
+
+/* x = Smooth(A{level},M{level},L,U,b,x0,niter,0) (Implementation/Smooth.m:1-24),
+ * level 1-based as the reference's cell index; vectors are that level's free vectors. */
+int tmg_smooth(tmg_solver *h, int level, const double *b, const double *x0, int niter, double *x);
+
+/* y = A{level} * x (the matrix products of Implementation/SolveMG.m:26,45 and
+ * Implementation/VCycle.m:24). Level 1-based; free vectors of that level. */
+int tmg_apply(tmg_solver *h, int level, const double *x, double *y);
+
+/* xf = P{level} * xc and xc = P{level}' * xf with P from
+ * [P,freedofsc] = Prolongation(nxf,nyf,nzf,nxc,nyc,nzc,nc,freedofsf)
+ * (Implementation/Prolongation.m:1-57, used at Implementation/VCycle.m:25,41).
+ * level is the fine level (1-based); vectors are free vectors of the levels. */
+int tmg_prolong(tmg_solver *h, int level, const double *xc, double *xf);
+int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc);
+
+/* Level dimensions and free DOFs: nel3 = {nelx,nely,nelz} of the level,
+ * freedofs1 (optional, nfree entries) the 1-based free DOF list of the level
+ * (the freedofsc output of Prolongation.m). Level 1-based. */
+int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t *freedofs1);
+int tmg_ngrids(const tmg_solver *h);
+
+/* Fgrad = Grad(str,u,p,emin,k,x,Dofs,W,w,opfilter,mr,strDens), single
+ * resolution (Implementation/Grad.m:18-23,62). u: full-length displacement
+ * (3*nnodes, fixed DOFs 0); x: filtered densities (n_el); freedens: optional
+ * 0/1 element mask (str.freeDensG; NULL = all); filt: the filter (NULL =
+ * opfilter 0). Fgrad: n_el. All host pointers. */
+int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens,
+             const tmg_filter *filt, double *Fgrad);
+
+/* Density filter [W,w] = ElemNeighborhood(str,rmin,opfilter)
+ * (Implementation/ElemNeighborhood.m:1-100) for opfilter 1 or 2 on a
+ * structured grid, fixed: optional 0/1 element mask (str.fixedDens).
+ * tmg_filter_apply is xfil = ApplyFilter(x,W,w,opfilter,trans)
+ * (Implementation/ApplyFilter.m:1-21). Host vectors of n_el. */
+int tmg_filter_create(int nelx, int nely, int nelz, double el, double eh, double ew,
+                      double rmin, int opfilter, const uint8_t *fixed, tmg_filter **out);
+int tmg_filter_apply(tmg_filter *f, const double *x, int trans, double *y);
+/* w = full(sum(W,2)), the row sums of W (Implementation/ElemNeighborhood.m:98). */
+int tmg_filter_rowsum(const tmg_filter *f, double *w);
+void tmg_filter_destroy(tmg_filter *f);
+
+/* Instrumentation (no counterpart in the reference): average device time in
+ * milliseconds of one launch of a solver kernel, measured with HIP events on
+ * the handle's stream over nrep back-to-back launches on the handle's own
+ * buffers. which: 0 = fine-level operator apply in FP64 (the PCG's q = K d),
+ * 1 = fine-level Jacobi sweep in the preconditioner precision, 2 = level-2
+ * stencil Jacobi sweep in the preconditioner precision, 3 = one V-cycle
+ * (preconditioner application). Requires a prior tmg_update_density. */
+int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
+
+/* Message of the last failed call on h (or of the last failed tmg_setup when h is NULL). */
+const char *tmg_last_error(const tmg_solver *h);
+
+/* Frees the hierarchy and all its device memory. */
+void tmg_destroy(tmg_solver *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

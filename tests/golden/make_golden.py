@@ -1,0 +1,124 @@
+#!/usr/bin/env python3
+"""Mint the golden fixtures under tests/golden/ (run in the build container,
+where the reference tree is mounted; the GPU box never reads the reference).
+
+1. setup_fixtures.json -- problem-setup known answers from the reference's
+   own problem library (Implementation/Tests/*.mat, MATLAB v5 files read with
+   scipy.io.loadmat, which executes nothing): the support and load rows, and
+   the resulting free-DOF list (as count + SHA-256 of its 1-based int64 bytes)
+   and the non-zero entries of the load vector str.f. Only files whose loads
+   are point or line loads are kept (the generator does not restate area loads).
+2. small_cantilever.npz -- oracle outputs on an 8x4x4 cantilever (filtered
+   random densities, seed 11, 3 levels): u, PCG count, compliance and
+   gradient, each cross-checked here against an independent scipy solve.
+3. oracle_<grid>_seed<s>.json -- FP64 oracle compliance / iterations of the
+   larger benchmark workloads (with --large), used by bench.py to state each
+   GPU variant's error against the oracle.
+"""
+import argparse
+import glob
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "topopt-slp-3d_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+REF_TESTS = "/root/reference/Implementation/Tests"
+
+
+def fd_hash(fd1):
+    return hashlib.sha256(np.ascontiguousarray(fd1, dtype="<i8").tobytes()).hexdigest()
+
+
+def setup_fixtures():
+    import scipy.io as sio
+    out = []
+    for path in sorted(glob.glob(os.path.join(REF_TESTS, "*.mat"))):
+        d = sio.loadmat(path, squeeze_me=True, struct_as_record=False)
+        s = d["str"]
+        if "supp" not in s._fieldnames or "loads" not in s._fieldnames:
+            continue
+        loads = np.atleast_2d(np.asarray(s.loads, dtype=float))
+        if any(sum(r[2 * k + 1] > r[2 * k] for k in range(3)) > 1 for r in loads):
+            continue  # area load
+        supp = np.atleast_2d(np.asarray(s.supp, dtype=float))
+        fd = np.asarray(s.freedofs, dtype=np.int64).ravel()
+        f = np.asarray(s.f, dtype=float).ravel()
+        nz = np.nonzero(f)[0]
+        out.append({
+            "file": os.path.basename(path),
+            "nelx": int(s.nelx), "nely": int(s.nely), "nelz": int(s.nelz),
+            "l": float(s.l), "h": float(s.h), "w": float(s.w),
+            "supp": supp.tolist(), "loads": loads.tolist(),
+            "nfree": int(fd.size), "freedofs_sha256": fd_hash(fd),
+            "f_nonzero_index1": (nz + 1).tolist(), "f_nonzero_value": f[nz].tolist(),
+        })
+    with open(os.path.join(HERE, "setup_fixtures.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(f"setup_fixtures.json: {len(out)} problems")
+
+
+def small_cantilever():
+    import scipy.sparse.linalg as sla
+
+    import oracle
+    from tmg import problem as P
+    prob = P.cantilever(8, 4, 4)
+    x = P.random_density(8, 4, 4, 11)
+    o = oracle.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(3, 0.6)
+    u, it, rr = o.solve(prob.f)
+    K = o.csr("A", 0).tocsc()
+    ud = sla.spsolve(K, prob.f)
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
+    W, w = oracle.filter_matrix(8, 4, 4, 1.5, 2)
+    g0 = o.grad0(prob.full(u), x)
+    g = W.T @ (g0 / w)
+    np.savez(os.path.join(HERE, "small_cantilever.npz"), x=x, u=u, it=it, compliance=prob.f @ u, g0=g0, g=g,
+             freedofs=prob.freedofs, f=prob.f)
+    print(f"small_cantilever.npz: it={it} C={prob.f @ u:.12e}")
+
+
+def large(grid, seed, levels):
+    import time
+
+    import oracle
+    from tmg import problem as P
+    nelx, nely, nelz = grid
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed) if seed is not None else np.full(prob.nelem, 0.3)
+    t0 = time.time()
+    o = oracle.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(levels, 0.6)
+    t1 = time.time()
+    u, it, rr = o.solve(prob.f)
+    t2 = time.time()
+    r = prob.f - o.spmv(0, u)
+    rec = {"grid": list(grid), "seed": seed, "levels": levels, "nfree": int(prob.nfree), "it": int(it),
+           "relres": float(rr), "true_relres": float(np.linalg.norm(r) / np.linalg.norm(prob.f)),
+           "compliance": float(prob.f @ u), "u_norm": float(np.linalg.norm(u)),
+           "u_tip_mean_y": float(np.mean(u[np.nonzero(prob.f)[0]])),
+           "setup_s": t1 - t0, "solve_s": t2 - t1, "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+    name = f"oracle_{nelx}x{nely}x{nelz}_seed{seed}.json"
+    with open(os.path.join(HERE, name), "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print(name, rec)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--large", action="store_true")
+    a = ap.parse_args()
+    setup_fixtures()
+    small_cantilever()
+    if a.large:
+        large((96, 48, 48), 2, 4)
+        large((192, 96, 96), 3, 5)

@@ -1,0 +1,216 @@
+"""GPU parity of libtmg against the FP64 oracle (the reference's assembled
+GMG-PCG path restated on the CPU), through the C-ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+from tmg import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _setup(prob, x, L, precision="fp64", **kw):
+    import tmg
+    s = tmg.Solver(prob, L, precision=precision, **kw)
+    s.update_density(x)
+    return s
+
+
+def _oracle(oracle_mod, prob, x, L):
+    o = oracle_mod.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(L, 0.6)
+    return o
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+SMALL = [(8, 4, 4, 3, 11), (16, 8, 8, 3, 12), (12, 8, 4, 2, 13)]
+
+
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", SMALL)
+def test_level_operators_match_oracle(oracle_mod, nelx, nely, nelz, L, seed):
+    _say(f"level operators {nelx}x{nely}x{nelz} L={L}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    s = _setup(prob, x, L)
+    rng = np.random.default_rng(seed)
+    for lev in range(1, L + 1):
+        nel, fd = s.level_info(lev)
+        assert nel == o.level_dims(lev - 1)
+        np.testing.assert_array_equal(fd - 1, o.level_free(lev - 1))
+        v = rng.standard_normal(fd.size)
+        yo = o.spmv(lev - 1, v)
+        assert _rel(s.apply(lev, v), yo) < 1e-13, lev
+        # smoother (Implementation/Smooth.m) from a random x0, 2 sweeps
+        b = rng.standard_normal(fd.size)
+        assert _rel(s.smooth(lev, b, v, 2), o.smooth(lev - 1, b, v, 2)) < 1e-12
+        if lev < L:
+            Pm = o.csr("P", lev - 1)
+            vc = rng.standard_normal(Pm.shape[1])
+            assert _rel(s.prolong(lev, vc), Pm @ vc) < 1e-15
+            vf = rng.standard_normal(Pm.shape[0])
+            assert _rel(s.restrict(lev, vf), Pm.T @ vf) < 1e-14
+
+
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", SMALL)
+def test_vcycle_matches_oracle(oracle_mod, nelx, nely, nelz, L, seed):
+    _say(f"vcycle {nelx}x{nely}x{nelz}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    s = _setup(prob, x, L)
+    b = np.random.default_rng(seed).standard_normal(prob.nfree)
+    # the coarse solve is an explicit FP64 inverse vs the oracle's triangular
+    # solves: agreement to the coarse system's conditioning times eps
+    assert _rel(s.vcycle(b), o.vcycle(b)) < 1e-8
+
+
+def _check_solution(oracle_mod, o, prob, u, it_gpu, it_or, u_or, tol_u):
+    r = prob.f - o.spmv(0, u)
+    relres = np.linalg.norm(r) / np.linalg.norm(prob.f)
+    C, Co = prob.f @ u, prob.f @ u_or
+    _say(f"  it {it_gpu} (oracle {it_or}) true relres {relres:.3e} compliance rel err {abs(C - Co) / abs(Co):.3e}")
+    assert relres <= 1.2e-8
+    assert abs(C - Co) / abs(Co) < 1e-7
+    assert _rel(u, u_or) < tol_u
+    return relres
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", SMALL)
+def test_solve_small(oracle_mod, precision, nelx, nely, nelz, L, seed):
+    _say(f"solve {precision} {nelx}x{nely}x{nelz}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    u_or, it_or, _ = o.solve(prob.f)
+    s = _setup(prob, x, L, precision)
+    u, it, rr, rc = s.solve(prob.f)
+    assert rc == 0 and rr <= 1e-8
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    else:
+        assert it <= it_or + 3
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+
+
+def test_config1_uniform_density(oracle_mod):
+    """48x24x24, rho = 0.3, 4 levels, line load; also the closed form
+    u(rho) = u(1) / (Emin + (1 - Emin) rho^3) for uniform densities."""
+    _say("config1")
+    prob = P.cantilever(48, 24, 24)
+    x = np.full(prob.nelem, 0.3)
+    o = _oracle(oracle_mod, prob, x, 4)
+    u_or, it_or, _ = o.solve(prob.f)
+    for prec in ("fp64", "mixed"):
+        s = _setup(prob, x, 4, prec)
+        u, it, rr, rc = s.solve(prob.f)
+        assert rc == 0
+        _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+        s1 = _setup(prob, np.ones(prob.nelem), 4, prec)
+        u1, _, _, _ = s1.solve(prob.f)
+        Emin = 1e-9
+        assert _rel(u, u1 / (Emin + (1 - Emin) * 0.3 ** 3)) < 1e-6
+
+
+def test_config2_fp64_and_mixed(oracle_mod):
+    """96x48x48 (~699k DOFs), filtered random densities (seed 2), 4 levels."""
+    _say("config2")
+    prob = P.cantilever(96, 48, 48)
+    x = P.random_density(96, 48, 48, 2)
+    o = _oracle(oracle_mod, prob, x, 4)
+    u_or, it_or, _ = o.solve(prob.f)
+    for prec in ("fp64", "mixed"):
+        s = _setup(prob, x, 4, prec)
+        u, it, rr, rc = s.solve(prob.f)
+        assert rc == 0
+        _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+        s.close()
+
+
+def test_bitwise_repeatable():
+    _say("repeatable")
+    prob = P.cantilever(16, 8, 8)
+    x = P.random_density(16, 8, 8, 5)
+    for prec in ("fp64", "mixed"):
+        s = _setup(prob, x, 3, prec)
+        u1, it1, _, _ = s.solve(prob.f)
+        u2, it2, _, _ = s.solve(prob.f)
+        s2 = _setup(prob, x, 3, prec)
+        u3, it3, _, _ = s2.solve(prob.f)
+        assert it1 == it2 == it3
+        assert np.array_equal(u1, u2) and np.array_equal(u1, u3)
+
+
+def test_warm_start_and_not_converged(oracle_mod):
+    """Warm start x0 (pcgp.opu0, Implementation/StrSLP.m:329-333) and the
+    silent stop at maxiterMG (Implementation/SolveMG.m:30)."""
+    _say("warm start")
+    prob = P.cantilever(16, 8, 8)
+    x = P.random_density(16, 8, 8, 6)
+    s = _setup(prob, x, 3)
+    u, it, rr, rc = s.solve(prob.f)
+    u2, it2, rr2, rc2 = s.solve(prob.f, x0=u)
+    assert rc2 == 0 and it2 <= 2
+    s1 = _setup(prob, x, 3, maxiter=1)
+    _, it1, rr1, rc1 = s1.solve(prob.f)
+    assert rc1 == 1 and it1 == 2 and rr1 > 1e-8
+    s0 = _setup(prob, x, 3, tol=0.0, maxiter=3)
+    _, it0, _, rc0 = s0.solve(prob.f)
+    assert rc0 == 1 and it0 == 4
+
+
+def test_grad_and_filter_match_oracle(oracle_mod):
+    _say("grad/filter")
+    import tmg
+    nelx, nely, nelz = 16, 8, 8
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, 7)
+    o = _oracle(oracle_mod, prob, x, 3)
+    u_or, _, _ = o.solve(prob.f)
+    ufull = prob.full(u_or)
+    s = _setup(prob, x, 3)
+    for opf, rmin in ((2, 1.5), (1, 2.2), (2, 2.5)):
+        W, w = oracle_mod.filter_matrix(nelx, nely, nelz, rmin, opf)
+        f = tmg.Filter(nelx, nely, nelz, rmin, opf)
+        assert _rel(f.rowsum(), w) < 1e-15
+        v = np.random.default_rng(opf).random(prob.nelem)
+        assert _rel(f.apply(v), (W @ v) / w) < 1e-14
+        assert _rel(f.apply(v, trans=True), W.T @ (v / w)) < 1e-14
+        g0 = o.grad0(ufull, x)
+        assert _rel(s.grad(ufull, x), g0) < 1e-12
+        assert _rel(s.grad(ufull, x, filt=f), W.T @ (g0 / w)) < 1e-12
+    fixed = np.zeros(prob.nelem, dtype=np.uint8)
+    fixed[::7] = 1
+    W, w = oracle_mod.filter_matrix(nelx, nely, nelz, 1.5, 2, fixed=fixed)
+    f = tmg.Filter(nelx, nely, nelz, 1.5, 2, fixed=fixed)
+    v = np.random.default_rng(9).random(prob.nelem)
+    assert _rel(f.apply(v), (W @ v) / w) < 1e-14
+    freedens = np.ones(prob.nelem, dtype=np.uint8)
+    freedens[::5] = 0
+    assert _rel(s.grad(ufull, x, freedens=freedens), o.grad0(ufull, x, freedens)) < 1e-12
+
+
+def test_error_paths():
+    _say("error paths")
+    import tmg
+    prob = P.cantilever(8, 4, 4)
+    s = tmg.Solver(prob, 3)
+    with pytest.raises(tmg.TmgError) as e:
+        s.solve(prob.f)   # no operator yet
+    assert e.value.code == -6
+    with pytest.raises(tmg.TmgError):
+        s.update_density(-np.ones(prob.nelem))
+    s.update_density(np.full(prob.nelem, 0.5))
+    with pytest.raises(tmg.TmgError):
+        s.apply(9, np.zeros(3))

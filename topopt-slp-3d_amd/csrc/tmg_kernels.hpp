@@ -1,0 +1,756 @@
+// tmg_kernels.hpp -- gfx950 kernels of the multigrid equilibrium solver.
+//
+// Structured grid of Q1 hexahedra, node-major vectors with the three
+// displacement components interleaved (dof = 3*node + c), full length with
+// fixed DOFs held at zero. node(ix,iy,iz) = iz*nx*ny + ix*ny + iy,
+// elem(ex,ey,ez) = ez*nelx*nely + ex*nely + ey, element-local node
+// a = dy + 2*dx + 4*dz (the reference's numbering, Implementation/EnterData.m:130,
+// Implementation/DegreesOfFreedom.m:24-54, Implementation/ElemStiffMatrix.m:331-332).
+//
+// Every reduction is two-stage and in a fixed order, and no kernel scatters
+// with atomics, so results are bitwise reproducible run to run and
+// independent of what else runs on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmg {
+
+constexpr int TPB = 256;
+
+// PCG state kept on the device so that an iteration needs no host round trip.
+struct PcgScalars {
+    double rho, gamma, alpha, beta, dq, rr, nf, nr, tol;
+    int it, maxiter, done, status;
+};
+
+enum { EPI_PLAIN = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_DOT = 3 };
+
+// ---------------------------------------------------------------------------
+// Block reduction of one double per thread into partial[block], fixed order.
+// ---------------------------------------------------------------------------
+__device__ inline void block_reduce_store(double v, double *partial, int64_t slot) {
+    __shared__ double red[TPB / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) s += red[w];
+        partial[slot] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fine-level operator: element-by-element y = sum_e E_e Ke x_e, gathered per
+// node (no scatter). Each thread owns one (ix,iy) column of nodes and walks a
+// chunk of z planes with a 3x3x3-node register window, so each x value is
+// loaded once per column step and every Ke entry is a wave-uniform scalar.
+// ---------------------------------------------------------------------------
+// Arguments: x input vector (full, fixed = 0); E element moduli; K0 the
+// 24x24 row-major element stiffness for E = 1; mask the free-DOF mask; b the
+// right-hand side (RESID, JACOBI); invD = omega / diag (0 at fixed DOFs,
+// JACOBI); out the result; partial per-block partial sums of x . (A x) (DOT);
+// done skips the launch once a PCG has converged. All pointers are
+// non-aliasing, which lets every Ke entry live in scalar registers.
+template <typename T, int EPI>
+__global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int zchunk, const T *__restrict__ x,
+                                             const T *__restrict__ E, const T *__restrict__ K0,
+                                             const uint8_t *__restrict__ mask, const T *__restrict__ b,
+                                             const T *__restrict__ invD, T *__restrict__ out,
+                                             double *__restrict__ partial, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t p0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const bool act = p0 < plane;
+    const int64_t p = act ? p0 : plane - 1;
+    const int ix = (int)(p / ny);
+    const int iy = (int)(p - (int64_t)ix * ny);
+    const int z0 = blockIdx.y * zchunk;
+    const int z1 = min(z0 + zchunk, nz);
+
+    // Neighbour indices are clamped into the grid, so every load is in
+    // bounds and needs no predicate; a clamped (nonexistent) neighbour only
+    // belongs to nonexistent elements, whose modulus is forced to zero below.
+    int cx[3], cy[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        cx[k] = min(max(ix - 1 + k, 0), nx - 1);
+        cy[k] = min(max(iy - 1 + k, 0), ny - 1);
+    }
+    int ecx[2], ecy[2];
+    T ewxy[2][2];  // 1 if the element column exists, else 0
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        ecx[o] = min(max(ix - 1 + o, 0), nelx - 1);
+        ecy[o] = min(max(iy - 1 + o, 0), nely - 1);
+    }
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+            ewxy[ox][oy] = (ix - 1 + ox >= 0 && ix - 1 + ox < nelx && iy - 1 + oy >= 0 && iy - 1 + oy < nely) ? T(1) : T(0);
+
+    T xw[3][3][3][3];  // [z][x][y][c] around (ix, iy, iz)
+    T ew[2][2][2];     // [z][x][y] elements around the node
+
+    auto load_plane = [&](int zz, int s) {
+        const int64_t zb = (int64_t)min(max(zz, 0), nz - 1) * plane;
+#pragma unroll
+        for (int xx = 0; xx < 3; ++xx)
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy) {
+                const int64_t nd = zb + (int64_t)cx[xx] * ny + cy[yy];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xw[s][xx][yy][c] = x[3 * nd + c];
+            }
+    };
+    auto load_elems = [&](int ez, int s) {
+        const T vz = (ez >= 0 && ez < nelz) ? T(1) : T(0);
+        const int64_t zb = (int64_t)min(max(ez, 0), nelz - 1) * nelx * nely;
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy)
+                ew[s][ox][oy] = E[zb + (int64_t)ecx[ox] * nely + ecy[oy]] * (vz * ewxy[ox][oy]);
+    };
+
+    load_plane(z0 - 1, 0);
+    load_plane(z0, 1);
+    load_elems(z0 - 1, 0);
+    double dacc = 0.0;
+    // Ke is read through the constant address space: the loads are scalar
+    // (s_load into SGPRs) and every VALU FMA takes its Ke operand from there.
+    typedef const __attribute__((address_space(4))) T *cptr;
+    cptr kp = (cptr)K0;
+    for (int iz = z0; iz < z1; ++iz) {
+        load_plane(iz + 1, 2);
+        load_elems(iz, 1);
+        T y[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+            for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 2; ++oy) {
+                    // this node is local node ai of the element at offset (ox,oy,oz)
+                    const int ai = (1 - oy) + 2 * (1 - ox) + 4 * (1 - oz);
+                    // rows 3*ai+c of Ke are consumed contiguously (24 scalars each)
+                    const T Ee = ew[oz][ox][oy];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        // Re-materialise the Ke base per row, ordered after the
+                        // previous row's result, so one row (24 scalars) is live
+                        // at a time: otherwise the loop-invariant scalar loads
+                        // are hoisted en bloc and spill to VGPR lanes.
+                        asm volatile("" : "+s"(kp) : "v"(y[(c + 2) % 3]));
+                        T t = T(0);
+#pragma unroll
+                        for (int j = 0; j < 24; ++j) {
+                            const int bn = j / 3, d = j % 3;
+                            const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
+                            t += kp[(3 * ai + c) * 24 + j] * xw[oz + bz][ox + bx][oy + by][d];
+                        }
+                        y[c] += Ee * t;
+                    }
+                }
+        if (act) {
+            const int64_t node = (int64_t)iz * plane + p;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t dof = 3 * node + c;
+                const bool m = mask[dof] != 0;
+                if (EPI == EPI_PLAIN) {
+                    out[dof] = m ? y[c] : T(0);
+                } else if (EPI == EPI_RESID) {
+                    out[dof] = m ? (b[dof] - y[c]) : T(0);
+                } else if (EPI == EPI_JACOBI) {
+                    out[dof] = m ? (xw[1][1][1][c] + invD[dof] * (b[dof] - y[c])) : T(0);
+                } else {
+                    const T v = m ? y[c] : T(0);
+                    out[dof] = v;
+                    dacc += (double)xw[1][1][1][c] * (double)v;
+                }
+            }
+        }
+#pragma unroll
+        for (int xx = 0; xx < 3; ++xx)
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    xw[0][xx][yy][c] = xw[1][xx][yy][c];
+                    xw[1][xx][yy][c] = xw[2][xx][yy][c];
+                }
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) ew[0][ox][oy] = ew[1][ox][oy];
+    }
+    if (EPI == EPI_DOT) block_reduce_store(dacc, partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-level operator: assembled 27-point block stencil (Galerkin P'AP).
+// Layout is structure-of-arrays blocked by wavefront: coefficient
+// (k*9 + c*3 + d) of node i sits at S[(i/64)*243*64 + (k*9 + c*3 + d)*64 + i%64],
+// k = neighbour offset (dz+1)*9 + (dx+1)*3 + (dy+1), so a wave's loads of one
+// coefficient are one contiguous 64-element run and every offset is a
+// compile-time constant.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int64_t sidx(int64_t i, int coef) { return (i >> 6) * (243 * 64) + coef * 64 + (i & 63); }
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(TPB) void k_stencil(int nx, int ny, int nz, const T *__restrict__ x,
+                                                 const T *__restrict__ S, const uint8_t *__restrict__ mask,
+                                                 const T *__restrict__ b, const T *__restrict__ invD,
+                                                 T *__restrict__ out, double *__restrict__ partial,
+                                                 const int *__restrict__ done) {
+    if (done && *done) return;
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t nn = plane * nz;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const bool act = i < nn;
+    double dacc = 0.0;
+    if (act) {
+        const int iz = (int)(i / plane);
+        const int ix = (int)((i / ny) % nx);
+        const int iy = (int)(i % ny);
+        T y[3] = {T(0), T(0), T(0)};
+        const T *Sw = S + (i >> 6) * (243 * 64) + (i & 63);
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int64_t jzb = (int64_t)min(max(iz + dz, 0), nz - 1) * plane;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int dx = q / 3 - 1, dy = q % 3 - 1;
+                const int k = (dz + 1) * 9 + q;
+                // clamped neighbour: its stencil coefficients are exactly zero
+                const int jx = min(max(ix + dx, 0), nx - 1), jy = min(max(iy + dy, 0), ny - 1);
+                const int64_t j = jzb + (int64_t)jx * ny + jy;
+                T xv[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) xv[d] = x[3 * j + d];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) y[c] += Sw[(k * 9 + c * 3 + d) * 64] * xv[d];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t dof = 3 * i + c;
+            const bool m = mask[dof] != 0;
+            if (EPI == EPI_PLAIN) {
+                out[dof] = m ? y[c] : T(0);
+            } else if (EPI == EPI_RESID) {
+                out[dof] = m ? (b[dof] - y[c]) : T(0);
+            } else if (EPI == EPI_JACOBI) {
+                out[dof] = m ? (x[dof] + invD[dof] * (b[dof] - y[c])) : T(0);
+            } else {
+                const T v = m ? y[c] : T(0);
+                out[dof] = v;
+                dacc += (double)x[dof] * (double)v;
+            }
+        }
+    }
+    if (EPI == EPI_DOT) block_reduce_store(dacc, partial, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Transfers: trilinear prolongation P (vertex-centred, weights 1, 1/2, 1/4,
+// 1/8, Implementation/Prolongation.m:15-37) restricted to free DOFs, and
+// restriction P'.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_restrict(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
+                                                  const T *__restrict__ rf, const uint8_t *__restrict__ cmask,
+                                                  T *__restrict__ bc, const int *done) {
+    if (done && *done) return;
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t cn = (int64_t)cnx * cny * cnz;
+    if (I >= cn) return;
+    const int Iz = (int)(I / ((int64_t)cnx * cny)), Ix = (int)((I / cny) % cnx), Iy = (int)(I % cny);
+    const int64_t fplane = (int64_t)fnx * fny;
+    T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
+        const int fz = 2 * Iz + dz, fx = 2 * Ix + dx, fy = 2 * Iy + dy;
+        if (fz < 0 || fz >= fnz || fx < 0 || fx >= fnx || fy < 0 || fy >= fny) continue;
+        const T w = T(1.0 / (double)(1 << (dz * dz + dx * dx + dy * dy)));
+        const int64_t j = (int64_t)fz * fplane + (int64_t)fx * fny + fy;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += w * rf[3 * j + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) bc[3 * I + c] = cmask[3 * I + c] ? acc[c] : T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_prolong_add(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
+                                                     const T *__restrict__ ec, const uint8_t *__restrict__ fmask,
+                                                     T *__restrict__ xf, int add, const int *done) {
+    if (done && *done) return;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t fn = (int64_t)fnx * fny * fnz;
+    if (i >= fn) return;
+    const int iz = (int)(i / ((int64_t)fnx * fny)), ix = (int)((i / fny) % fnx), iy = (int)(i % fny);
+    const int64_t cplane = (int64_t)cnx * cny;
+    T acc[3] = {T(0), T(0), T(0)};
+    const int zl = iz >> 1, zh = (iz + 1) >> 1, xl = ix >> 1, xh = (ix + 1) >> 1, yl = iy >> 1, yh = (iy + 1) >> 1;
+    const T wz = (zl == zh) ? T(1) : T(0.5), wx = (xl == xh) ? T(1) : T(0.5), wy = (yl == yh) ? T(1) : T(0.5);
+    for (int Z = zl; Z <= zh; ++Z)
+        for (int X = xl; X <= xh; ++X)
+            for (int Y = yl; Y <= yh; ++Y) {
+                const int64_t J = (int64_t)Z * cplane + (int64_t)X * cny + Y;
+                const T w = wz * wx * wy;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] += w * ec[3 * J + c];
+            }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t dof = 3 * i + c;
+        const T v = fmask[dof] ? acc[c] : T(0);
+        xf[dof] = add ? xf[dof] + v : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_jacobi_zero(int64_t n, const T *__restrict__ b, const T *__restrict__ invD,
+                                                     T *__restrict__ x, const int *done) {
+    if (done && *done) return;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) x[i] = invD[i] * b[i];
+}
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(TPB) void k_convert(int64_t n, const TS *__restrict__ s, TD *__restrict__ d, const int *done) {
+    if (done && *done) return;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) d[i] = (TD)s[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_fill(int64_t n, T v, T *__restrict__ d) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
+// partial[block] = sum a_i * b_i over a grid-stride range
+template <typename TA, typename TB>
+__global__ __launch_bounds__(TPB) void k_dot(int64_t n, const TA *__restrict__ a, const TB *__restrict__ b,
+                                             double *partial, const int *done) {
+    if (done && *done) return;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+        s += (double)a[i] * (double)b[i];
+    block_reduce_store(s, partial, blockIdx.x);
+}
+
+// d = beta d + s  (d = s on the first iteration, Implementation/SolveMG.m:39-44)
+template <typename TS>
+__global__ __launch_bounds__(TPB) void k_dir_update(int64_t n, const TS *__restrict__ s, double *__restrict__ d,
+                                                    const PcgScalars *sc) {
+    if (sc->done) return;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    if (sc->it == 1) d[i] = (double)s[i];
+    else d[i] = sc->beta * d[i] + (double)s[i];
+}
+
+// x += alpha d; r -= alpha q; partial ||r||^2  (Implementation/SolveMG.m:47-50)
+__global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                   const double *__restrict__ d, const double *__restrict__ q,
+                                                   double *partial, const PcgScalars *sc) {
+    if (sc->done) return;
+    const double al = sc->alpha;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        x[i] = x[i] + al * d[i];
+        const double ri = r[i] - al * q[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    block_reduce_store(s, partial, blockIdx.x);
+}
+
+enum { FIN_INIT = 0, FIN_RHO = 1, FIN_ALPHA = 2, FIN_RR = 3 };
+
+// Single-block, fixed-order final reduction and the scalar PCG recurrences.
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_finalize(int np, const double *partial, int np2, const double *partial2,
+                                                   PcgScalars *sc) {
+    if (MODE != FIN_INIT && sc->done) return;
+    __shared__ double red[1024];
+    __shared__ double red2[1024];
+    double s = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < np; i += 1024) s += partial[i];
+    if (MODE == FIN_INIT)
+        for (int i = threadIdx.x; i < np2; i += 1024) s2 += partial2[i];
+    red[threadIdx.x] = s;
+    red2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x] += red[threadIdx.x + w];
+            red2[threadIdx.x] += red2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double tot = red[0];
+        if (MODE == FIN_INIT) {           // r = f - A x0; nr = ||r||/||f||; it = 1
+            sc->nf = sqrt(red2[0]);
+            sc->rr = tot;
+            sc->nr = sqrt(tot) / sc->nf;
+            sc->it = 1;
+            sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
+        } else if (MODE == FIN_RHO) {     // rho = r' s; beta = rho / gamma2
+            sc->rho = tot;
+            sc->beta = (sc->it == 1) ? 0.0 : tot / sc->gamma;
+        } else if (MODE == FIN_ALPHA) {   // alpha = rho / (d' q)
+            sc->dq = tot;
+            sc->alpha = sc->rho / tot;
+        } else {                          // gamma2 = rho; nr = ||r||/||f||; it++
+            sc->rr = tot;
+            sc->gamma = sc->rho;
+            sc->nr = sqrt(tot) / sc->nf;
+            sc->it = sc->it + 1;
+            sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest grid: gather free DOFs, y = Ainv b (FP64, one wave per row),
+// scatter back. Ainv is the inverse of the coarsest Galerkin operator from its
+// Cholesky factorization.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int64_t *__restrict__ idx, const T *__restrict__ v,
+                                                double *__restrict__ out, const int *done) {
+    if (done && *done) return;
+    const int64_t q = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) out[q] = (double)v[idx[q]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_scatter_full(int64_t ndof, const int64_t *__restrict__ cidx,
+                                                      const double *__restrict__ v, T *__restrict__ out, const int *done) {
+    if (done && *done) return;
+    const int64_t d = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (d >= ndof) return;
+    const int64_t q = cidx[d];
+    out[d] = q >= 0 ? (T)v[q] : T(0);
+}
+
+__global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,
+                                              double *__restrict__ y, const int *done) {
+    if (done && *done) return;
+    const int64_t row = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double *Ar = A + row * n;
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s += Ar[j] * x[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[row] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Setup: moduli, diagonals, Galerkin element matrices, stencils, dense coarse.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_modulus(int64_t ne, const double *__restrict__ x, double E0, double emin,
+                                                 double p, double *__restrict__ E64, float *__restrict__ E32) {
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e >= ne) return;
+    const double v = emin + (E0 - emin) * pow(x[e], p);   // Implementation/GlobalStiffMatrix.m:17
+    E64[e] = v;
+    if (E32) E32[e] = (float)v;
+}
+
+// Jacobi at the fine level: invD = 1 / ((1/omega) diag(K)), diag(K) gathered
+// element by element (the recipe of Implementation/WOK/DiagPrec.m).
+__global__ __launch_bounds__(TPB) void k_diag_fine(int nelx, int nely, int nelz, const double *__restrict__ E,
+                                                   const double *__restrict__ K0, const uint8_t *__restrict__ mask,
+                                                   double omega, double *__restrict__ inv64, float *__restrict__ inv32) {
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= (int64_t)nx * ny * nz) return;
+    const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
+    double dg[3] = {0, 0, 0};
+    for (int oz = 0; oz < 2; ++oz)
+        for (int ox = 0; ox < 2; ++ox)
+            for (int oy = 0; oy < 2; ++oy) {
+                const int ex = ix - 1 + ox, ey = iy - 1 + oy, ez = iz - 1 + oz;
+                if (ex < 0 || ex >= nelx || ey < 0 || ey >= nely || ez < 0 || ez >= nelz) continue;
+                const double Ee = E[(int64_t)ez * nelx * nely + (int64_t)ex * nely + ey];
+                const int ai = (1 - oy) + 2 * (1 - ox) + 4 * (1 - oz);
+                for (int c = 0; c < 3; ++c) dg[c] += Ee * K0[(3 * ai + c) * 24 + 3 * ai + c];
+            }
+    for (int c = 0; c < 3; ++c) {
+        const int64_t dof = 3 * i + c;
+        const double v = mask[dof] ? 1.0 / ((1.0 / omega) * dg[c]) : 0.0;   // M = (1/omega) diag(A)
+        inv64[dof] = v;
+        if (inv32) inv32[dof] = (float)v;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_diag_stencil(int64_t nn, const double *__restrict__ S,
+                                                      const uint8_t *__restrict__ mask, double omega,
+                                                      double *__restrict__ inv64, float *__restrict__ inv32) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nn) return;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t dof = 3 * i + c;
+        const double dg = S[sidx(i, 13 * 9 + c * 3 + c)];
+        const double v = mask[dof] ? 1.0 / ((1.0 / omega) * dg) : 0.0;
+        inv64[dof] = v;
+        if (inv32) inv32[dof] = (float)v;
+    }
+}
+
+__device__ inline double tri_w(int off) {  // 1-D trilinear weight of a fine node at offset off from a coarse vertex
+    return off == 0 ? 1.0 : ((off == 1 || off == -1) ? 0.5 : 0.0);
+}
+
+// Galerkin element matrices of the first coarse level from the fine moduli:
+// Ahat_E = sum_k T_k' D_k (E_k Ke) D_k T_k over the 8 children k, D_k the
+// fine free-DOF mask, T_k the trilinear interpolation of the coarse element
+// onto child k. One thread per (coarse element, 3x3 output block); output
+// structure-of-arrays Ahat[(row*24 + col)*ne + E].
+__global__ __launch_bounds__(TPB) void k_galerkin_fine(int cnelx, int cnely, int cnelz, const double *__restrict__ E,
+                                                       const double *__restrict__ K0, const uint8_t *__restrict__ fmask,
+                                                       double *__restrict__ Ahat) {
+    const int64_t cne = (int64_t)cnelx * cnely * cnelz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= cne * 64) return;
+    const int blk = (int)(t / cne);
+    const int64_t Ec = t - (int64_t)blk * cne;
+    const int A = blk >> 3, B = blk & 7;
+    const int Ez = (int)(Ec / ((int64_t)cnelx * cnely)), Ex = (int)((Ec / cnely) % cnelx), Ey = (int)(Ec % cnely);
+    const int Ay = A & 1, Ax = (A >> 1) & 1, Az = A >> 2;
+    const int By = B & 1, Bx = (B >> 1) & 1, Bz = B >> 2;
+    const int fnelx = 2 * cnelx, fnely = 2 * cnely;
+    const int fnx = fnelx + 1, fny = fnely + 1;
+    double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 8; ++k) {
+        const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+        const double Ek = E[(int64_t)(2 * Ez + kz) * fnelx * fnely + (int64_t)(2 * Ex + kx) * fnely + (2 * Ey + ky)];
+#pragma unroll
+        for (int an = 0; an < 8; ++an) {
+            const int ay = an & 1, ax = (an >> 1) & 1, az = an >> 2;
+            const double tA = tri_w(kx + ax - 2 * Ax) * tri_w(ky + ay - 2 * Ay) * tri_w(kz + az - 2 * Az);
+            if (tA == 0.0) continue;
+            const int64_t na = (int64_t)(2 * Ez + kz + az) * fnx * fny + (int64_t)(2 * Ex + kx + ax) * fny + (2 * Ey + ky + ay);
+            double ma[3];
+            for (int c = 0; c < 3; ++c) ma[c] = fmask[3 * na + c] ? tA * Ek : 0.0;
+#pragma unroll
+            for (int bn = 0; bn < 8; ++bn) {
+                const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
+                const double tB = tri_w(kx + bx - 2 * Bx) * tri_w(ky + by - 2 * By) * tri_w(kz + bz - 2 * Bz);
+                if (tB == 0.0) continue;
+                const int64_t nb = (int64_t)(2 * Ez + kz + bz) * fnx * fny + (int64_t)(2 * Ex + kx + bx) * fny + (2 * Ey + ky + by);
+                double mb[3];
+                for (int d = 0; d < 3; ++d) mb[d] = fmask[3 * nb + d] ? tB : 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) R[c][d] += ma[c] * mb[d] * K0[(3 * an + c) * 24 + 3 * bn + d];
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) Ahat[(int64_t)((3 * A + c) * 24 + 3 * B + d) * cne + Ec] = R[c][d];
+}
+
+// Galerkin element matrices of level l+1 from those of level l (same
+// recursion with the child matrices read from Ahat_f and the level-l mask).
+__global__ __launch_bounds__(TPB) void k_galerkin_coarse(int cnelx, int cnely, int cnelz, const double *__restrict__ Ahat_f,
+                                                         const uint8_t *__restrict__ fmask, double *__restrict__ Ahat) {
+    const int64_t cne = (int64_t)cnelx * cnely * cnelz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= cne * 64) return;
+    const int blk = (int)(t / cne);
+    const int64_t Ec = t - (int64_t)blk * cne;
+    const int A = blk >> 3, B = blk & 7;
+    const int Ez = (int)(Ec / ((int64_t)cnelx * cnely)), Ex = (int)((Ec / cnely) % cnelx), Ey = (int)(Ec % cnely);
+    const int Ay = A & 1, Ax = (A >> 1) & 1, Az = A >> 2;
+    const int By = B & 1, Bx = (B >> 1) & 1, Bz = B >> 2;
+    const int fnelx = 2 * cnelx, fnely = 2 * cnely, fnelz = 2 * cnelz;
+    const int64_t fne = (int64_t)fnelx * fnely * fnelz;
+    const int fnx = fnelx + 1, fny = fnely + 1;
+    double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 8; ++k) {
+        const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
+        const int64_t ek = (int64_t)(2 * Ez + kz) * fnelx * fnely + (int64_t)(2 * Ex + kx) * fnely + (2 * Ey + ky);
+        for (int an = 0; an < 8; ++an) {
+            const int ay = an & 1, ax = (an >> 1) & 1, az = an >> 2;
+            const double tA = tri_w(kx + ax - 2 * Ax) * tri_w(ky + ay - 2 * Ay) * tri_w(kz + az - 2 * Az);
+            if (tA == 0.0) continue;
+            const int64_t na = (int64_t)(2 * Ez + kz + az) * fnx * fny + (int64_t)(2 * Ex + kx + ax) * fny + (2 * Ey + ky + ay);
+            double ma[3];
+            for (int c = 0; c < 3; ++c) ma[c] = fmask[3 * na + c] ? tA : 0.0;
+            for (int bn = 0; bn < 8; ++bn) {
+                const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
+                const double tB = tri_w(kx + bx - 2 * Bx) * tri_w(ky + by - 2 * By) * tri_w(kz + bz - 2 * Bz);
+                if (tB == 0.0) continue;
+                const int64_t nb = (int64_t)(2 * Ez + kz + bz) * fnx * fny + (int64_t)(2 * Ex + kx + bx) * fny + (2 * Ey + ky + by);
+                double mb[3];
+                for (int d = 0; d < 3; ++d) mb[d] = fmask[3 * nb + d] ? tB : 0.0;
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d)
+                        R[c][d] += ma[c] * mb[d] * Ahat_f[(int64_t)((3 * an + c) * 24 + 3 * bn + d) * fne + ek];
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) Ahat[(int64_t)((3 * A + c) * 24 + 3 * B + d) * cne + Ec] = R[c][d];
+}
+
+// Assembled block stencil of a level from its element matrices, masked by the
+// level's free-DOF mask on both sides. One thread per (node, neighbour).
+template <typename T32>
+__global__ __launch_bounds__(TPB) void k_stencil_assemble(int nelx, int nely, int nelz, const double *__restrict__ Ahat,
+                                                          const uint8_t *__restrict__ mask, double *__restrict__ S64,
+                                                          T32 *__restrict__ S32) {
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const int64_t nn = (int64_t)nx * ny * nz, ne = (int64_t)nelx * nely * nelz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= nn * 27) return;
+    const int k = (int)(t / nn);
+    const int64_t i = t - (int64_t)k * nn;
+    const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
+    const int jz = iz + k / 9 - 1, jx = ix + (k / 3) % 3 - 1, jy = iy + k % 3 - 1;
+    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const bool v = jz >= 0 && jz < nz && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+    if (v) {
+        const int64_t j = (int64_t)jz * nx * ny + (int64_t)jx * ny + jy;
+        for (int ez = max(iz, jz) - 1; ez <= min(iz, jz); ++ez)
+            for (int ex = max(ix, jx) - 1; ex <= min(ix, jx); ++ex)
+                for (int ey = max(iy, jy) - 1; ey <= min(iy, jy); ++ey) {
+                    if (ez < 0 || ez >= nelz || ex < 0 || ex >= nelx || ey < 0 || ey >= nely) continue;
+                    const int64_t e = (int64_t)ez * nelx * nely + (int64_t)ex * nely + ey;
+                    const int ai = (iy - ey) + 2 * (ix - ex) + 4 * (iz - ez);
+                    const int aj = (jy - ey) + 2 * (jx - ex) + 4 * (jz - ez);
+                    for (int c = 0; c < 3; ++c)
+                        for (int d = 0; d < 3; ++d) acc[c][d] += Ahat[(int64_t)((3 * ai + c) * 24 + 3 * aj + d) * ne + e];
+                }
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                if (!(mask[3 * i + c] && mask[3 * j + d])) acc[c][d] = 0.0;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) {
+            const int64_t o = sidx(i, k * 9 + c * 3 + d);
+            S64[o] = acc[c][d];
+            if (S32) S32[o] = (T32)acc[c][d];
+        }
+}
+
+// Dense coarsest operator (free DOFs, row-major) from the coarsest stencil.
+__global__ __launch_bounds__(TPB) void k_dense_from_stencil(int nx, int ny, int nz, const double *__restrict__ S,
+                                                            const int64_t *__restrict__ cidx, int64_t n,
+                                                            double *__restrict__ Ad) {
+    const int64_t nn = (int64_t)nx * ny * nz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= nn * 27) return;
+    const int k = (int)(t / nn);
+    const int64_t i = t - (int64_t)k * nn;
+    const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
+    const int jz = iz + k / 9 - 1, jx = ix + (k / 3) % 3 - 1, jy = iy + k % 3 - 1;
+    if (jz < 0 || jz >= nz || jx < 0 || jx >= nx || jy < 0 || jy >= ny) return;
+    const int64_t j = (int64_t)jz * nx * ny + (int64_t)jx * ny + jy;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t row = cidx[3 * i + c];
+        if (row < 0) continue;
+        for (int d = 0; d < 3; ++d) {
+            const int64_t col = cidx[3 * j + d];
+            if (col < 0) continue;
+            Ad[row * n + col] = S[sidx(i, k * 9 + c * 3 + d)];
+        }
+    }
+}
+
+// Fill the lower triangle of a column-major symmetric matrix from the upper
+// one (rocSOLVER potri writes one triangle).
+__global__ __launch_bounds__(TPB) void k_symmetrize_upper(int64_t n, double *__restrict__ A) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t r = t % n, c = t / n;  // column-major (r, c)
+    if (r > c) A[c * n + r] = A[r * n + c];
+}
+
+// ---------------------------------------------------------------------------
+// Compliance gradient and density filter.
+// ---------------------------------------------------------------------------
+// g0_e = -p x_e^(p-1) (E0 - emin) u_e' Ke u_e (Implementation/Grad.m:18-23)
+__global__ __launch_bounds__(TPB) void k_grad0(int nelx, int nely, int nelz, const double *__restrict__ u,
+                                               const double *__restrict__ x, const uint8_t *__restrict__ freedens,
+                                               const double *__restrict__ K0, double p, double E0, double emin,
+                                               double *__restrict__ g0) {
+    const int64_t ne = (int64_t)nelx * nely * nelz;
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e >= ne) return;
+    if (freedens && !freedens[e]) { g0[e] = 0.0; return; }
+    const int nx = nelx + 1, ny = nely + 1;
+    const int ez = (int)(e / ((int64_t)nelx * nely)), ex = (int)((e / nely) % nelx), ey = (int)(e % nely);
+    double ue[24];
+#pragma unroll
+    for (int an = 0; an < 8; ++an) {
+        const int dy = an & 1, dx = (an >> 1) & 1, dz = an >> 2;
+        const int64_t nd = (int64_t)(ez + dz) * nx * ny + (int64_t)(ex + dx) * ny + (ey + dy);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * an + c] = u[3 * nd + c];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) t += K0[i * 24 + j] * ue[j];
+        s += ue[i] * t;
+    }
+    g0[e] = -p * pow(x[e], p - 1.0) * (E0 - emin) * s;
+}
+
+// Filter row sums w and weighted neighbourhood products on the element grid.
+// Offsets/weights depend only on the offset (Implementation/ElemNeighborhood.m:25-39);
+// fixed elements are decoupled with unit self weight (:90-96).
+__global__ __launch_bounds__(TPB) void k_filter(int nelx, int nely, int nelz, int noff, const int *__restrict__ off,
+                                                const double *__restrict__ wt, const uint8_t *__restrict__ fixed,
+                                                const double *__restrict__ x, const double *__restrict__ scale_in,
+                                                const double *__restrict__ scale_out, double *__restrict__ y) {
+    const int64_t ne = (int64_t)nelx * nely * nelz;
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e >= ne) return;
+    const int ez = (int)(e / ((int64_t)nelx * nely)), ex = (int)((e / nely) % nelx), ey = (int)(e % nely);
+    const bool ef = fixed && fixed[e];
+    double s = 0.0;
+    for (int k = 0; k < noff; ++k) {
+        const int dx = off[3 * k], dy = off[3 * k + 1], dz = off[3 * k + 2];
+        const int jx = ex + dx, jy = ey + dy, jz = ez + dz;
+        if (jx < 0 || jx >= nelx || jy < 0 || jy >= nely || jz < 0 || jz >= nelz) continue;
+        const int64_t j = (int64_t)jz * nelx * nely + (int64_t)jx * nely + jy;
+        double w;
+        if (j == e) w = ef ? 1.0 : wt[k];
+        else {
+            if (ef || (fixed && fixed[j])) continue;
+            w = wt[k];
+        }
+        const double xv = x ? x[j] : 1.0;
+        s += w * (scale_in ? xv / scale_in[j] : xv);
+    }
+    y[e] = scale_out ? s / scale_out[e] : s;
+}
+
+}  // namespace tmg

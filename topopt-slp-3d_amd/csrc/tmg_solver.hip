@@ -1,0 +1,971 @@
+// tmg_solver.hip -- host side of libtmg: device-resident GMG hierarchy,
+// Galerkin refresh, V-cycle orchestration, device-driven PCG and the C-ABI
+// declared in include/tmg.h.
+#include "tmg_kernels.hpp"
+#include "tmg.h"
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace tmg;
+
+namespace {
+
+constexpr int MAXL = 12;
+thread_local std::string g_setup_err;
+
+struct DevLevel {
+    int nelx = 0, nely = 0, nelz = 0, nx = 0, ny = 0, nz = 0;
+    int64_t nn = 0, ne = 0, ndof = 0, nfree = 0;
+    uint8_t *mask = nullptr;
+    double *S64 = nullptr;
+    float *S32 = nullptr;
+    double *inv64 = nullptr;
+    float *inv32 = nullptr;
+    void *b = nullptr, *x = nullptr, *t = nullptr;  // V-cycle vectors, preconditioner precision
+    std::vector<int64_t> free_full;                 // compact -> full dof (0-based)
+    int64_t *d_free = nullptr;                      // device copy of free_full
+    int64_t *d_cidx = nullptr;                      // full dof -> compact index or -1
+};
+
+}  // namespace
+
+struct tmg_solver {
+    int nelx, nely, nelz;
+    double el, eh, ew, E0, nu, penal, emin;
+    tmg_mg_params mg;
+    int prec = TMG_FP64;
+    int L = 0;
+    DevLevel lev[MAXL];
+    double ke[576];
+    double *K0_64 = nullptr;
+    float *K0_32 = nullptr;
+    double *E64 = nullptr;
+    float *E32 = nullptr;
+    double *xdens = nullptr;
+    double *Ahat[2] = {nullptr, nullptr};
+    // coarsest grid
+    int64_t nc = 0;
+    double *Ainv = nullptr, *cb = nullptr, *cx = nullptr;
+    int *d_info = nullptr;
+    rocblas_handle rb = nullptr;
+    // PCG
+    double *px = nullptr, *pr = nullptr, *pd = nullptr, *pq = nullptr, *pf = nullptr;
+    double *partial = nullptr, *partial2 = nullptr;
+    int npartial = 0;
+    PcgScalars *d_sc = nullptr;
+    PcgScalars *h_sc = nullptr;
+    int zchunk = 4;
+    hipStream_t st = nullptr;
+    bool have_op = false;
+    std::string err;
+};
+
+namespace {
+
+#define HIPX(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return TMG_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+#define TRY(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ < 0) return rc_;       \
+    } while (0)
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <typename T>
+int dalloc(tmg_solver *h, T **p, int64_t n) {
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)std::max<int64_t>(n, 1));
+    if (e != hipSuccess) {
+        h->err = "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) + " bytes failed: " + hipGetErrorString(e);
+        return TMG_ERR_OOM;
+    }
+    return 0;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// operator applies
+// ---------------------------------------------------------------------------
+template <typename T> const T *K0of(tmg_solver *h);
+template <> const double *K0of<double>(tmg_solver *h) { return h->K0_64; }
+template <> const float *K0of<float>(tmg_solver *h) { return h->K0_32; }
+template <typename T> const T *Eof(tmg_solver *h);
+template <> const double *Eof<double>(tmg_solver *h) { return h->E64; }
+template <> const float *Eof<float>(tmg_solver *h) { return h->E32; }
+template <typename T> const T *Sof(const DevLevel &l);
+template <> const double *Sof<double>(const DevLevel &l) { return l.S64; }
+template <> const float *Sof<float>(const DevLevel &l) { return l.S32; }
+template <typename T> const T *invof(const DevLevel &l);
+template <> const double *invof<double>(const DevLevel &l) { return l.inv64; }
+template <> const float *invof<float>(const DevLevel &l) { return l.inv32; }
+
+// Number of blocks an apply of this level launches (partial-sum slots).
+int64_t apply_blocks(tmg_solver *h, int l) {
+    const DevLevel &lv = h->lev[l];
+    if (l == 0) {
+        int64_t pb = nblk((int64_t)lv.nx * lv.ny);
+        return pb * ((lv.nz + h->zchunk - 1) / h->zchunk);
+    }
+    return nblk(lv.nn);
+}
+
+template <typename T, int EPI>
+void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial, const int *done) {
+    const DevLevel &lv = h->lev[l];
+    if (l == 0) {
+        dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + h->zchunk - 1) / h->zchunk);
+        hipLaunchKernelGGL((k_ebe<T, EPI>), grid, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, h->zchunk, x, Eof<T>(h),
+                           K0of<T>(h), lv.mask, b, invof<T>(lv), out, partial, done);
+    } else {
+        hipLaunchKernelGGL((k_stencil<T, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, x,
+                           Sof<T>(lv), lv.mask, b, invof<T>(lv), out, partial, done);
+    }
+}
+
+template <typename T>
+void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
+    DevLevel &lc = h->lev[h->L - 1];
+    hipLaunchKernelGGL((k_gather<T>), dim3(nblk(h->nc)), dim3(TPB), 0, h->st, h->nc, lc.d_free, b, h->cb, done);
+    hipLaunchKernelGGL(k_gemv, dim3((unsigned)((h->nc + 3) / 4)), dim3(TPB), 0, h->st, h->nc, h->Ainv, h->cb, h->cx, done);
+    hipLaunchKernelGGL((k_scatter_full<T>), dim3(nblk(lc.ndof)), dim3(TPB), 0, h->st, lc.ndof, lc.d_cidx, h->cx, x, done);
+}
+
+// One V(nu1,nu2) cycle from a zero initial guess at level l
+// (Implementation/VCycle.m:23-43). Returns the buffer holding the result.
+template <typename T>
+T *vcycle(tmg_solver *h, int l, const T *b, const int *done) {
+    DevLevel &lv = h->lev[l];
+    DevLevel &lc = h->lev[l + 1];
+    T *xa = (T *)lv.x, *xb = (T *)lv.t;
+    const int nu1 = h->mg.smoothiter1, nu2 = h->mg.smoothiter2;
+    if (nu1 >= 1) {
+        // first sweep from x0 = 0: x = M \ b (Implementation/Smooth.m:17 with x = 0)
+        hipLaunchKernelGGL((k_jacobi_zero<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, b, invof<T>(lv), xa, done);
+        for (int s = 1; s < nu1; ++s) {
+            apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
+            std::swap(xa, xb);
+        }
+    } else {
+        hipLaunchKernelGGL((k_fill<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, T(0), xa);
+    }
+    apply<T, EPI_RESID>(h, l, xa, b, xb, nullptr, done);                          // rh = b - A x
+    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st,    // rH = P' rh
+                       lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, xb, lc.mask, (T *)lc.b, done);
+    T *ec;
+    if (l + 1 == h->L - 1) {
+        coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);                      // eH = R \ (R' \ rH)
+        ec = (T *)lc.x;
+    } else {
+        ec = vcycle<T>(h, l + 1, (const T *)lc.b, done);
+    }
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st,  // x += P eH
+                       lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, (const T *)ec, lv.mask, xa, 1, done);
+    for (int s = 0; s < nu2; ++s) {
+        apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
+        std::swap(xa, xb);
+    }
+    return xa;
+}
+
+// Preconditioner application s = B r for the PCG; r is the FP64 residual.
+// Returns s as a typed pointer through one of the two out-params.
+void precondition(tmg_solver *h, const double *r, const double **s64, const float **s32, const int *done) {
+    DevLevel &l0 = h->lev[0];
+    if (h->prec == TMG_MIXED) {
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, (float *)l0.b, done);
+        *s32 = vcycle<float>(h, 0, (const float *)l0.b, done);
+        *s64 = nullptr;
+    } else {
+        *s64 = vcycle<double>(h, 0, r, done);
+        *s32 = nullptr;
+    }
+}
+
+// One PCG iteration of Implementation/SolveMG.m:30-52, all scalars on device.
+void pcg_iteration(tmg_solver *h) {
+    const int *done = &h->d_sc->done;
+    const int64_t n = h->lev[0].ndof;
+    const double *s64;
+    const float *s32;
+    precondition(h, h->pr, &s64, &s32, done);
+    const int nb = h->npartial;
+    const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), nb);
+    if (s32) {
+        hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
+    } else {
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
+    }
+    hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+    if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
+    else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
+    apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);          // q = A d, d'q
+    hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, (int)apply_blocks(h, 0), h->partial, 0,
+                       nullptr, h->d_sc);
+    hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial, h->d_sc);
+    hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+}
+
+int check_launch(tmg_solver *h) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        h->err = std::string("kernel launch: ") + hipGetErrorString(e);
+        return TMG_ERR_HIP;
+    }
+    return 0;
+}
+
+// Device-resident solve on full-length FP64 vectors px (x0 in, x out) and pf.
+int solve_resident(tmg_solver *h, int *it, double *relres) {
+    if (!h->have_op) {
+        h->err = "no operator: call tmg_update_density first";
+        return TMG_ERR_STATE;
+    }
+    const int64_t n = h->lev[0].ndof;
+    HIPX(hipStreamSynchronize(h->st));  // h_sc is reused as the staging buffer
+    memset(h->h_sc, 0, sizeof(PcgScalars));
+    h->h_sc->tol = h->mg.tolMG;
+    h->h_sc->maxiter = h->mg.maxiterMG;
+    HIPX(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, h->st));
+    // d = 0 so that the first direction update is exact
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd);
+    // r = f - A x0 (Implementation/SolveMG.m:26-27)
+    apply<double, EPI_RESID>(h, 0, h->px, h->pf, h->pr, nullptr, nullptr);
+    const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), h->npartial);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pr, h->partial, nullptr);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pf, h->pf, h->partial2, nullptr);
+    hipLaunchKernelGGL(k_finalize<FIN_INIT>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, (int)gdot,
+                       h->partial2, h->d_sc);
+    TRY(check_launch(h));
+    int chunk = 4;
+    for (;;) {
+        for (int k = 0; k < chunk; ++k) pcg_iteration(h);
+        TRY(check_launch(h));
+        HIPX(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->st));
+        HIPX(hipStreamSynchronize(h->st));
+        if (h->h_sc->done) break;
+        chunk = std::min(chunk * 2, 32);
+    }
+    *it = h->h_sc->it;
+    *relres = h->h_sc->nr;
+    if (!std::isfinite(h->h_sc->nr)) {
+        h->err = "PCG breakdown (non-finite residual)";
+        return TMG_ERR_NOT_SPD;
+    }
+    return (h->h_sc->nr > h->mg.tolMG) ? TMG_NOT_CONVERGED : TMG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// setup / refresh
+// ---------------------------------------------------------------------------
+int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
+    DevLevel &lv = h->lev[l];
+    lv.nx = lv.nelx + 1; lv.ny = lv.nely + 1; lv.nz = lv.nelz + 1;
+    lv.nn = (int64_t)lv.nx * lv.ny * lv.nz;
+    lv.ne = (int64_t)lv.nelx * lv.nely * lv.nelz;
+    lv.ndof = 3 * lv.nn;
+    lv.free_full.clear();
+    std::vector<int64_t> cidx(lv.ndof, -1);
+    for (int64_t d = 0; d < lv.ndof; ++d)
+        if (mask[d]) { cidx[d] = (int64_t)lv.free_full.size(); lv.free_full.push_back(d); }
+    lv.nfree = (int64_t)lv.free_full.size();
+    TRY(dalloc(h, &lv.mask, lv.ndof));
+    TRY(dalloc(h, &lv.d_free, lv.nfree));
+    TRY(dalloc(h, &lv.d_cidx, lv.ndof));
+    HIPX(hipMemcpy(lv.mask, mask.data(), lv.ndof, hipMemcpyHostToDevice));
+    if (lv.nfree) HIPX(hipMemcpy(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice));
+    const size_t vs = h->prec == TMG_MIXED ? sizeof(float) : sizeof(double);
+    TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * vs)));
+    TRY(dalloc(h, (char **)&lv.x, (int64_t)(lv.ndof * vs)));
+    TRY(dalloc(h, (char **)&lv.t, (int64_t)(lv.ndof * vs)));
+    TRY(dalloc(h, &lv.inv64, lv.ndof));
+    if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.inv32, lv.ndof));
+    if (l > 0) {
+        const int64_t ns = ((lv.nn + 63) / 64) * 64 * 243;  // wave-blocked stencil storage
+        TRY(dalloc(h, &lv.S64, ns));
+        if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.S32, ns));
+    }
+    return 0;
+}
+
+void destroy_all(tmg_solver *h) {
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (int l = 0; l < MAXL; ++l) {
+        DevLevel &lv = h->lev[l];
+        dfree(lv.mask); dfree(lv.S64); dfree(lv.S32); dfree(lv.inv64); dfree(lv.inv32);
+        dfree(lv.b); dfree(lv.x); dfree(lv.t); dfree(lv.d_free); dfree(lv.d_cidx);
+    }
+    dfree(h->K0_64); dfree(h->K0_32); dfree(h->E64); dfree(h->E32); dfree(h->xdens);
+    dfree(h->Ahat[0]); dfree(h->Ahat[1]);
+    dfree(h->Ainv); dfree(h->cb); dfree(h->cx); dfree(h->d_info);
+    dfree(h->px); dfree(h->pr); dfree(h->pd); dfree(h->pq); dfree(h->pf);
+    dfree(h->partial); dfree(h->partial2); dfree(h->d_sc);
+    if (h->h_sc) (void)hipHostFree(h->h_sc);
+    h->h_sc = nullptr;
+    if (h->rb) rocblas_destroy_handle(h->rb);
+    h->rb = nullptr;
+    if (h->st) (void)hipStreamDestroy(h->st);
+    h->st = nullptr;
+}
+
+// Element stiffness by exact 2x2x2 Gauss integration of B'DB on the box
+// (equal to the closed form of Implementation/ElemStiffMatrix.m, local DOF
+// order 3*(dy + 2 dx + 4 dz) + c).
+void element_stiffness(double el, double eh, double ew, double nu, double *k) {
+    const double a = 0.5 * el, b = 0.5 * eh, c = 0.5 * ew;
+    const double g = 1.0 / std::sqrt(3.0);
+    const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
+    std::fill(k, k + 576, 0.0);
+    for (int q = 0; q < 8; ++q) {
+        const double xi = (q & 2 ? g : -g), et = (q & 1 ? g : -g), ze = (q & 4 ? g : -g);
+        double dN[8][3];
+        for (int an = 0; an < 8; ++an) {
+            const double sy = (an & 1) ? 1 : -1, sx = (an & 2) ? 1 : -1, sz = (an & 4) ? 1 : -1;
+            dN[an][0] = sx * (1 + sy * et) * (1 + sz * ze) / (8 * a);
+            dN[an][1] = sy * (1 + sx * xi) * (1 + sz * ze) / (8 * b);
+            dN[an][2] = sz * (1 + sx * xi) * (1 + sy * et) / (8 * c);
+        }
+        const double w = a * b * c;
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) {
+                const double *gi = dN[i], *gj = dN[j];
+                const double dot = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+                for (int ci = 0; ci < 3; ++ci)
+                    for (int cj = 0; cj < 3; ++cj) {
+                        // (B'DB)_{ci,cj} = lam gi_ci gj_cj + mu gi_cj gj_ci + mu delta (gi . gj)
+                        double v = lam * gi[ci] * gj[cj] + mu * gi[cj] * gj[ci];
+                        if (ci == cj) v += mu * dot;
+                        k[(3 * i + ci) * 24 + 3 * j + cj] += w * v;
+                    }
+            }
+    }
+    for (int i = 0; i < 24; ++i)
+        for (int j = i + 1; j < 24; ++j) k[j * 24 + i] = k[i * 24 + j];
+}
+
+int refresh(tmg_solver *h) {
+    DevLevel &l0 = h->lev[0];
+    const double om = h->mg.omega;
+    hipLaunchKernelGGL(k_modulus, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.ne, h->xdens, h->E0, h->emin, h->penal,
+                       h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
+    hipLaunchKernelGGL(k_diag_fine, dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->E64, h->K0_64,
+                       l0.mask, om, l0.inv64, h->prec == TMG_MIXED ? l0.inv32 : nullptr);
+    for (int l = 1; l < h->L; ++l) {
+        DevLevel &lv = h->lev[l];
+        double *dst = h->Ahat[(l - 1) & 1];
+        if (l == 1) {
+            hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(lv.ne * 64)), dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz,
+                               h->E64, h->K0_64, l0.mask, dst);
+        } else {
+            hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(lv.ne * 64)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
+                               lv.nelz, h->Ahat[l & 1], h->lev[l - 1].mask, dst);
+        }
+        hipLaunchKernelGGL((k_stencil_assemble<float>), dim3(nblk(lv.nn * 27)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
+                           lv.nelz, dst, lv.mask, lv.S64, h->prec == TMG_MIXED ? lv.S32 : nullptr);
+        hipLaunchKernelGGL(k_diag_stencil, dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nn, lv.S64, lv.mask, om, lv.inv64,
+                           h->prec == TMG_MIXED ? lv.inv32 : nullptr);
+    }
+    // coarsest: dense operator, Cholesky, inverse (Implementation/SetupGMG.m:51-60)
+    DevLevel &lc = h->lev[h->L - 1];
+    const int64_t nc = h->nc;
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc * nc, 0.0, h->Ainv);
+    hipLaunchKernelGGL(k_dense_from_stencil, dim3(nblk(lc.nn * 27)), dim3(TPB), 0, h->st, lc.nx, lc.ny, lc.nz, lc.S64,
+                       lc.d_cidx, nc, h->Ainv);
+    TRY(check_launch(h));
+    if (nc > 0) {
+        rocblas_status s1 = rocsolver_dpotrf(h->rb, rocblas_fill_upper, (rocblas_int)nc, h->Ainv, (rocblas_int)nc, h->d_info);
+        int info = 0;
+        HIPX(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->st));
+        HIPX(hipStreamSynchronize(h->st));
+        if (s1 != rocblas_status_success) {
+            h->err = "rocsolver_dpotrf failed (status " + std::to_string((int)s1) + ")";
+            return TMG_ERR_HIP;
+        }
+        if (info != 0) {
+            h->err = "coarse Galerkin operator is not positive definite (potrf info " + std::to_string(info) + ")";
+            return TMG_ERR_NOT_SPD;
+        }
+        rocblas_status s2 = rocsolver_dpotri(h->rb, rocblas_fill_upper, (rocblas_int)nc, h->Ainv, (rocblas_int)nc, h->d_info);
+        if (s2 != rocblas_status_success) {
+            h->err = "rocsolver_dpotri failed (status " + std::to_string((int)s2) + ")";
+            return TMG_ERR_HIP;
+        }
+        hipLaunchKernelGGL(k_symmetrize_upper, dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc, h->Ainv);
+    }
+    TRY(check_launch(h));
+    HIPX(hipStreamSynchronize(h->st));
+    h->have_op = true;
+    return 0;
+}
+
+// host free-vector helpers
+int upload_free(tmg_solver *h, int l, const double *hv, double *d_full) {
+    DevLevel &lv = h->lev[l];
+    double *tmp = nullptr;
+    TRY(dalloc(h, &tmp, lv.nfree));
+    HIPX(hipMemcpyAsync(tmp, hv, sizeof(double) * lv.nfree, hipMemcpyHostToDevice, h->st));
+    hipLaunchKernelGGL((k_scatter_full<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, lv.d_cidx, tmp,
+                       d_full, nullptr);
+    HIPX(hipStreamSynchronize(h->st));
+    dfree(tmp);
+    return 0;
+}
+
+int download_free(tmg_solver *h, int l, const double *d_full, double *hv) {
+    DevLevel &lv = h->lev[l];
+    double *tmp = nullptr;
+    TRY(dalloc(h, &tmp, lv.nfree));
+    hipLaunchKernelGGL((k_gather<double>), dim3(nblk(lv.nfree)), dim3(TPB), 0, h->st, lv.nfree, lv.d_free, d_full, tmp,
+                       nullptr);
+    HIPX(hipMemcpyAsync(hv, tmp, sizeof(double) * lv.nfree, hipMemcpyDeviceToHost, h->st));
+    HIPX(hipStreamSynchronize(h->st));
+    dfree(tmp);
+    return 0;
+}
+
+int set_err(tmg_solver *h, int rc, const std::string &m) {
+    h->err = m;
+    return rc;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, tmg_solver **out) {
+    g_setup_err.clear();
+    if (!prob || !mg || !out) { g_setup_err = "null argument"; return TMG_ERR_ARG; }
+    *out = nullptr;
+    if (prob->nelx <= 0 || prob->nely <= 0 || prob->nelz <= 0 || !(prob->el > 0) || !(prob->eh > 0) || !(prob->ew > 0) ||
+        !(prob->nu > -1.0 && prob->nu < 0.5) || !(prob->E0 > 0) || prob->emin < 0 || prob->nfree < 0 ||
+        (prob->nfree > 0 && !prob->freedofs)) {
+        g_setup_err = "invalid problem description";
+        return TMG_ERR_ARG;
+    }
+    if (mg->cycle != 0 || mg->smoother != 0 || mg->opchol != 0) {
+        g_setup_err = "only cycle=0 (V), smoother=0 (Jacobi), opchol=0 (coarse Cholesky) are supported";
+        return TMG_ERR_UNSUPPORTED;
+    }
+    if (mg->ngrids < 2 || mg->ngrids > MAXL || mg->smoothiter1 < 0 || mg->smoothiter2 < 0 || !(mg->omega > 0) ||
+        mg->maxiterMG < 0 || !(mg->tolMG >= 0)) {
+        g_setup_err = "invalid multigrid parameters";
+        return TMG_ERR_ARG;
+    }
+    const int d = 1 << (mg->ngrids - 1);
+    if (prob->nelx % d || prob->nely % d || prob->nelz % d) {
+        g_setup_err = "element counts must be divisible by 2^(ngrids-1) (Implementation/StartStrSLP.m:107-124)";
+        return TMG_ERR_ARG;
+    }
+    if (precision != TMG_FP64 && precision != TMG_MIXED) { g_setup_err = "unknown precision"; return TMG_ERR_ARG; }
+
+    tmg_solver *h = new tmg_solver();
+    h->nelx = prob->nelx; h->nely = prob->nely; h->nelz = prob->nelz;
+    h->el = prob->el; h->eh = prob->eh; h->ew = prob->ew;
+    h->E0 = prob->E0; h->nu = prob->nu; h->penal = prob->penal; h->emin = prob->emin;
+    h->mg = *mg;
+    h->prec = precision;
+    h->L = mg->ngrids;
+    auto fail = [&](int rc) {
+        g_setup_err = h->err;
+        destroy_all(h);
+        delete h;
+        return rc;
+    };
+    // level grids and free-DOF masks (Prolongation.m:38-46,54: coarse DOF free
+    // iff the co-located fine DOF is free); validated before any device call
+    std::vector<uint8_t> mask;
+    {
+        DevLevel &l0 = h->lev[0];
+        l0.nelx = prob->nelx; l0.nely = prob->nely; l0.nelz = prob->nelz;
+        const int64_t ndof = 3LL * (l0.nelx + 1) * (l0.nely + 1) * (l0.nelz + 1);
+        mask.assign(ndof, 0);
+        int64_t prev = 0;
+        for (int64_t i = 0; i < prob->nfree; ++i) {
+            const int64_t fd = prob->freedofs[i];
+            if (fd < 1 || fd > ndof || fd <= prev) {
+                h->err = "freedofs must be 1-based, in range and strictly increasing";
+                return fail(TMG_ERR_ARG);
+            }
+            prev = fd;
+            mask[fd - 1] = 1;
+        }
+    }
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        h->err = "hipStreamCreate failed";
+        return fail(TMG_ERR_HIP);
+    }
+    for (int l = 0; l < h->L; ++l) {
+        DevLevel &lv = h->lev[l];
+        if (l > 0) {
+            const DevLevel &lf = h->lev[l - 1];
+            lv.nelx = lf.nelx / 2; lv.nely = lf.nely / 2; lv.nelz = lf.nelz / 2;
+            const int cnx = lv.nelx + 1, cny = lv.nely + 1, cnz = lv.nelz + 1;
+            std::vector<uint8_t> cm(3LL * cnx * cny * cnz, 0);
+            for (int iz = 0; iz < cnz; ++iz)
+                for (int ix = 0; ix < cnx; ++ix)
+                    for (int iy = 0; iy < cny; ++iy) {
+                        const int64_t I = (int64_t)iz * cnx * cny + (int64_t)ix * cny + iy;
+                        const int64_t fi = (int64_t)(2 * iz) * lf.nx * lf.ny + (int64_t)(2 * ix) * lf.ny + 2 * iy;
+                        for (int c = 0; c < 3; ++c) cm[3 * I + c] = mask[3 * fi + c];
+                    }
+            mask.swap(cm);
+        }
+        int rc = alloc_level(h, l, mask);
+        if (rc) return fail(rc);
+    }
+    h->nc = h->lev[h->L - 1].nfree;
+    const DevLevel &l0 = h->lev[0];
+    element_stiffness(h->el, h->eh, h->ew, h->nu, h->ke);
+    float ke32[576];
+    for (int i = 0; i < 576; ++i) ke32[i] = (float)h->ke[i];
+    int rc = 0;
+    if ((rc = dalloc(h, &h->K0_64, 576)) || (rc = dalloc(h, &h->K0_32, 576)) || (rc = dalloc(h, &h->E64, l0.ne)) ||
+        (rc = dalloc(h, &h->E32, l0.ne)) || (rc = dalloc(h, &h->xdens, l0.ne)) ||
+        (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
+        (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
+        (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
+        (rc = dalloc(h, &h->px, l0.ndof)) || (rc = dalloc(h, &h->pr, l0.ndof)) || (rc = dalloc(h, &h->pd, l0.ndof)) ||
+        (rc = dalloc(h, &h->pq, l0.ndof)) || (rc = dalloc(h, &h->pf, l0.ndof)))
+        return fail(rc);
+    // z-chunk of the fine apply: enough blocks to fill the chip
+    {
+        const int64_t pb = nblk((int64_t)l0.nx * l0.ny);
+        int zc = (int)((l0.nz * pb + 4095) / 4096);
+        h->zchunk = std::max(1, std::min(zc, 16));
+    }
+    h->npartial = (int)std::max<int64_t>(apply_blocks(h, 0), 4096);
+    if ((rc = dalloc(h, &h->partial, h->npartial)) || (rc = dalloc(h, &h->partial2, h->npartial)) ||
+        (rc = dalloc(h, &h->d_sc, 1)))
+        return fail(rc);
+    if (hipHostMalloc((void **)&h->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) {
+        h->err = "hipHostMalloc failed";
+        return fail(TMG_ERR_OOM);
+    }
+    if (hipMemcpy(h->K0_64, h->ke, sizeof(h->ke), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->K0_32, ke32, sizeof(ke32), hipMemcpyHostToDevice) != hipSuccess) {
+        h->err = "hipMemcpy of Ke failed";
+        return fail(TMG_ERR_HIP);
+    }
+    if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
+        h->err = "rocblas_create_handle failed";
+        return fail(TMG_ERR_HIP);
+    }
+    rocblas_set_stream(h->rb, h->st);
+    *out = h;
+    return TMG_OK;
+}
+
+int tmg_update_density_device(tmg_solver *h, const double *d_x) {
+    if (!h || !d_x) return TMG_ERR_ARG;
+    HIPX(hipMemcpyAsync(h->xdens, d_x, sizeof(double) * h->lev[0].ne, hipMemcpyDeviceToDevice, h->st));
+    return refresh(h);
+}
+
+int tmg_update_density(tmg_solver *h, const double *x) {
+    if (!h || !x) return TMG_ERR_ARG;
+    for (int64_t e = 0; e < h->lev[0].ne; ++e)
+        if (!(x[e] >= 0.0) || !std::isfinite(x[e])) return set_err(h, TMG_ERR_ARG, "densities must be finite and >= 0");
+    HIPX(hipMemcpyAsync(h->xdens, x, sizeof(double) * h->lev[0].ne, hipMemcpyHostToDevice, h->st));
+    return refresh(h);
+}
+
+int tmg_solve_device(tmg_solver *h, double *d_x, const double *d_f, int *it, double *relres) {
+    if (!h || !d_x || !d_f) return TMG_ERR_ARG;
+    const int64_t n = h->lev[0].ndof;
+    HIPX(hipMemcpyAsync(h->px, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPX(hipMemcpyAsync(h->pf, d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    int iters = 0;
+    double rr = 0.0;
+    int rc = solve_resident(h, &iters, &rr);
+    if (rc < 0) return rc;
+    HIPX(hipMemcpyAsync(d_x, h->px, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    HIPX(hipStreamSynchronize(h->st));
+    if (it) *it = iters;
+    if (relres) *relres = rr;
+    return rc;
+}
+
+int tmg_solve(tmg_solver *h, const double *x0, const double *f, double *x_out, int *it, double *relres) {
+    if (!h || !x0 || !f || !x_out) return TMG_ERR_ARG;
+    TRY(upload_free(h, 0, x0, h->px));
+    TRY(upload_free(h, 0, f, h->pf));
+    int iters = 0;
+    double rr = 0.0;
+    int rc = solve_resident(h, &iters, &rr);
+    if (rc < 0) return rc;
+    TRY(download_free(h, 0, h->px, x_out));
+    if (it) *it = iters;
+    if (relres) *relres = rr;
+    return rc;
+}
+
+int tmg_vcycle(tmg_solver *h, const double *b, double *x) {
+    if (!h || !b || !x) return TMG_ERR_ARG;
+    if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
+    DevLevel &l0 = h->lev[0];
+    TRY(upload_free(h, 0, b, h->pr));
+    const double *s64;
+    const float *s32;
+    precondition(h, h->pr, &s64, &s32, nullptr);
+    if (s32) hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, s32, h->pq, nullptr);
+    else HIPX(hipMemcpyAsync(h->pq, s64, sizeof(double) * l0.ndof, hipMemcpyDeviceToDevice, h->st));
+    TRY(check_launch(h));
+    return download_free(h, 0, h->pq, x);
+}
+
+// Level-wise helpers work in FP64 on scratch copies of the level vectors.
+static int level_arg(tmg_solver *h, int level) {
+    if (!h) return TMG_ERR_ARG;
+    if (level < 1 || level > h->L) return set_err(h, TMG_ERR_ARG, "level out of range");
+    if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
+    return 0;
+}
+
+int tmg_apply(tmg_solver *h, int level, const double *x, double *y) {
+    TRY(level_arg(h, level));
+    const int l = level - 1;
+    DevLevel &lv = h->lev[l];
+    double *dx = nullptr, *dy = nullptr;
+    TRY(dalloc(h, &dx, lv.ndof));
+    TRY(dalloc(h, &dy, lv.ndof));
+    int rc = upload_free(h, l, x, dx);
+    if (!rc) {
+        apply<double, EPI_PLAIN>(h, l, dx, nullptr, dy, nullptr, nullptr);
+        rc = check_launch(h);
+    }
+    if (!rc) rc = download_free(h, l, dy, y);
+    dfree(dx); dfree(dy);
+    return rc;
+}
+
+int tmg_smooth(tmg_solver *h, int level, const double *b, const double *x0, int niter, double *x) {
+    TRY(level_arg(h, level));
+    if (niter < 0) return set_err(h, TMG_ERR_ARG, "niter < 0");
+    const int l = level - 1;
+    DevLevel &lv = h->lev[l];
+    double *db = nullptr, *xa = nullptr, *xb = nullptr;
+    TRY(dalloc(h, &db, lv.ndof));
+    TRY(dalloc(h, &xa, lv.ndof));
+    TRY(dalloc(h, &xb, lv.ndof));
+    int rc = upload_free(h, l, b, db);
+    if (!rc) rc = upload_free(h, l, x0, xa);
+    if (!rc) {
+        for (int s = 0; s < niter; ++s) {  // x = x + M \ (b - A x) (Implementation/Smooth.m:16-18)
+            apply<double, EPI_JACOBI>(h, l, xa, db, xb, nullptr, nullptr);
+            std::swap(xa, xb);
+        }
+        rc = check_launch(h);
+    }
+    if (!rc) rc = download_free(h, l, xa, x);
+    dfree(db); dfree(xa); dfree(xb);
+    return rc;
+}
+
+int tmg_prolong(tmg_solver *h, int level, const double *xc, double *xf) {
+    if (!h) return TMG_ERR_ARG;
+    if (level < 1 || level >= h->L) return set_err(h, TMG_ERR_ARG, "level out of range");
+    DevLevel &lf = h->lev[level - 1], &lc = h->lev[level];
+    double *dc = nullptr, *df = nullptr;
+    TRY(dalloc(h, &dc, lc.ndof));
+    TRY(dalloc(h, &df, lf.ndof));
+    int rc = upload_free(h, level, xc, dc);
+    if (!rc) {
+        hipLaunchKernelGGL((k_prolong_add<double>), dim3(nblk(lf.nn)), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
+                           lc.ny, lc.nz, dc, lf.mask, df, 0, nullptr);
+        rc = check_launch(h);
+    }
+    if (!rc) rc = download_free(h, level - 1, df, xf);
+    dfree(dc); dfree(df);
+    return rc;
+}
+
+int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc) {
+    if (!h) return TMG_ERR_ARG;
+    if (level < 1 || level >= h->L) return set_err(h, TMG_ERR_ARG, "level out of range");
+    DevLevel &lf = h->lev[level - 1], &lc = h->lev[level];
+    double *dc = nullptr, *df = nullptr;
+    TRY(dalloc(h, &dc, lc.ndof));
+    TRY(dalloc(h, &df, lf.ndof));
+    int rc = upload_free(h, level - 1, xf, df);
+    if (!rc) {
+        hipLaunchKernelGGL((k_restrict<double>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
+                           lc.ny, lc.nz, df, lc.mask, dc, nullptr);
+        rc = check_launch(h);
+    }
+    if (!rc) rc = download_free(h, level, dc, xc);
+    dfree(dc); dfree(df);
+    return rc;
+}
+
+int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t *freedofs1) {
+    if (!h) return TMG_ERR_ARG;
+    if (level < 1 || level > h->L) return set_err(h, TMG_ERR_ARG, "level out of range");
+    const DevLevel &lv = h->lev[level - 1];
+    if (nel3) { nel3[0] = lv.nelx; nel3[1] = lv.nely; nel3[2] = lv.nelz; }
+    if (nfree) *nfree = lv.nfree;
+    if (freedofs1)
+        for (int64_t q = 0; q < lv.nfree; ++q) freedofs1[q] = lv.free_full[q] + 1;
+    return 0;
+}
+
+int tmg_ngrids(const tmg_solver *h) { return h ? h->L : 0; }
+
+int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
+    if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 3) return TMG_ERR_ARG;
+    if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
+    if (which == 2 && h->L < 3) return set_err(h, TMG_ERR_ARG, "level 2 is the coarsest level here");
+    DevLevel &l0 = h->lev[0];
+    const bool mixed = h->prec == TMG_MIXED;
+    // deterministic non-trivial inputs: x = f-like ones pattern on free DOFs
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pd);
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pr);
+    auto launch = [&]() {
+        if (which == 0) {
+            apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, nullptr);
+        } else if (which == 1 || which == 2) {
+            const int l = which == 1 ? 0 : 1;
+            DevLevel &lv = h->lev[l];
+            if (mixed) apply<float, EPI_JACOBI>(h, l, (const float *)lv.x, (const float *)lv.b, (float *)lv.t, nullptr, nullptr);
+            else apply<double, EPI_JACOBI>(h, l, (const double *)lv.x, (const double *)lv.b, (double *)lv.t, nullptr, nullptr);
+        } else {
+            const double *s64;
+            const float *s32;
+            precondition(h, h->pr, &s64, &s32, nullptr);
+        }
+    };
+    // level vectors need finite values for the sweeps
+    for (int l = 0; l < 2 && l < h->L; ++l) {
+        DevLevel &lv = h->lev[l];
+        if (mixed) {
+            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 1.0f, (float *)lv.x);
+            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 1.0f, (float *)lv.b);
+        } else {
+            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 1.0, (double *)lv.x);
+            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 1.0, (double *)lv.b);
+        }
+    }
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    HIPX(hipEventRecord(e0, h->st));
+    for (int r = 0; r < nrep; ++r) launch();
+    HIPX(hipEventRecord(e1, h->st));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TRY(check_launch(h));
+    *avg_ms = (double)ms / nrep;
+    return TMG_OK;
+}
+
+const char *tmg_last_error(const tmg_solver *h) { return h ? h->err.c_str() : g_setup_err.c_str(); }
+
+void tmg_destroy(tmg_solver *h) {
+    if (!h) return;
+    destroy_all(h);
+    delete h;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Density filter and compliance gradient
+// ===========================================================================
+struct tmg_filter {
+    int nelx, nely, nelz, noff;
+    int *off = nullptr;
+    double *wt = nullptr;
+    uint8_t *fixed = nullptr;
+    double *w = nullptr;
+    hipStream_t st = nullptr;
+    std::string err;
+};
+
+namespace {
+
+void filter_free(tmg_filter *f) {
+    if (f->st) (void)hipStreamSynchronize(f->st);
+    dfree(f->off); dfree(f->wt); dfree(f->fixed); dfree(f->w);
+    if (f->st) (void)hipStreamDestroy(f->st);
+    f->st = nullptr;
+}
+
+// Neighbourhood offsets and weights (Implementation/ElemNeighborhood.m:10-39):
+// |dz| <= floor(rmin/ew); |dx| <= floor(sqrt(rmin^2 - dz^2)/el);
+// |dy| <= floor(sqrt(rmin^2 - dz^2 - dx^2)/eh); linear cone (rmin - d)/rmin
+// (opfilter 2) or Gaussian exp(-d^2 alpha) beta (opfilter 1); self weight 1
+// or beta (:60-64).
+void filter_offsets(double el, double eh, double ew, double rmin, int opfilter, std::vector<int> &off,
+                    std::vector<double> &wt) {
+    const int frz = (int)std::floor(rmin / ew);
+    const double rmin2 = rmin * rmin;
+    const double alpha = 1.0 / (2.0 * (rmin / 3.0) * (rmin / 3.0));
+    const double beta = 1.0 / (2.0 * M_PI * (rmin / 3.0));
+    for (int adz = 0; adz <= frz; ++adz) {
+        const double dz2 = (adz * ew) * (adz * ew);
+        const int relx = (int)std::floor(std::sqrt(rmin2 - dz2) / el);
+        for (int adx = 0; adx <= relx; ++adx) {
+            const double dx2 = (adx * el) * (adx * el);
+            const int rely = (int)std::floor(std::sqrt(rmin2 - dz2 - dx2) / eh);
+            for (int ady = 0; ady <= rely; ++ady) {
+                const double d2 = dz2 + dx2 + (ady * eh) * (ady * eh);
+                double w;
+                if (d2 == 0.0) w = opfilter == 1 ? beta : 1.0;
+                else w = opfilter == 1 ? std::exp(-d2 * alpha) * beta : (rmin - std::sqrt(d2)) / rmin;
+                for (int sz = (adz ? -1 : 1); sz <= 1; sz += 2)
+                    for (int sx = (adx ? -1 : 1); sx <= 1; sx += 2)
+                        for (int sy = (ady ? -1 : 1); sy <= 1; sy += 2) {
+                            off.push_back(sx * adx);
+                            off.push_back(sy * ady);
+                            off.push_back(sz * adz);
+                            wt.push_back(w);
+                        }
+            }
+        }
+    }
+}
+
+#define FHIPX(expr)                                                              \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            f->err = std::string(#expr) + ": " + hipGetErrorString(e_);          \
+            return TMG_ERR_HIP;                                                  \
+        }                                                                        \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int tmg_filter_create(int nelx, int nely, int nelz, double el, double eh, double ew, double rmin, int opfilter,
+                      const uint8_t *fixed, tmg_filter **out) {
+    g_setup_err.clear();
+    if (!out || nelx <= 0 || nely <= 0 || nelz <= 0 || !(el > 0) || !(eh > 0) || !(ew > 0) || !(rmin > 0) ||
+        (opfilter != 1 && opfilter != 2)) {
+        g_setup_err = "invalid filter arguments";
+        return TMG_ERR_ARG;
+    }
+    *out = nullptr;
+    tmg_filter *f = new tmg_filter();
+    f->nelx = nelx; f->nely = nely; f->nelz = nelz;
+    std::vector<int> off;
+    std::vector<double> wt;
+    filter_offsets(el, eh, ew, rmin, opfilter, off, wt);
+    f->noff = (int)wt.size();
+    const int64_t ne = (int64_t)nelx * nely * nelz;
+    auto bad = [&](const char *m) {
+        g_setup_err = m;
+        filter_free(f);
+        delete f;
+        return TMG_ERR_HIP;
+    };
+    if (hipStreamCreateWithFlags(&f->st, hipStreamNonBlocking) != hipSuccess) return bad("hipStreamCreate failed");
+    if (hipMalloc((void **)&f->off, sizeof(int) * off.size()) != hipSuccess ||
+        hipMalloc((void **)&f->wt, sizeof(double) * wt.size()) != hipSuccess ||
+        hipMalloc((void **)&f->w, sizeof(double) * ne) != hipSuccess)
+        return bad("hipMalloc failed");
+    if (fixed) {
+        if (hipMalloc((void **)&f->fixed, ne) != hipSuccess) return bad("hipMalloc failed");
+        if (hipMemcpy(f->fixed, fixed, ne, hipMemcpyHostToDevice) != hipSuccess) return bad("hipMemcpy failed");
+    }
+    if (hipMemcpy(f->off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->wt, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bad("hipMemcpy failed");
+    // w = row sums of W (Implementation/ElemNeighborhood.m:98)
+    hipLaunchKernelGGL(k_filter, dim3(nblk(ne)), dim3(TPB), 0, f->st, nelx, nely, nelz, f->noff, f->off, f->wt,
+                       f->fixed, nullptr, nullptr, nullptr, f->w);
+    if (hipStreamSynchronize(f->st) != hipSuccess) return bad("filter setup failed");
+    *out = f;
+    return TMG_OK;
+}
+
+static int filter_apply_dev(tmg_filter *f, const double *dx, int trans, double *dy, hipStream_t st) {
+    const int64_t ne = (int64_t)f->nelx * f->nely * f->nelz;
+    // trans = 0: (W x)./w ; trans = 1: W'(x./w) = W (x./w) (W is symmetric)
+    hipLaunchKernelGGL(k_filter, dim3(nblk(ne)), dim3(TPB), 0, st, f->nelx, f->nely, f->nelz, f->noff, f->off, f->wt,
+                       f->fixed, dx, trans ? f->w : nullptr, trans ? nullptr : f->w, dy);
+    return hipGetLastError() == hipSuccess ? 0 : TMG_ERR_HIP;
+}
+
+int tmg_filter_apply(tmg_filter *f, const double *x, int trans, double *y) {
+    if (!f || !x || !y) return TMG_ERR_ARG;
+    const int64_t ne = (int64_t)f->nelx * f->nely * f->nelz;
+    double *dx = nullptr, *dy = nullptr;
+    FHIPX(hipMalloc((void **)&dx, sizeof(double) * ne));
+    FHIPX(hipMalloc((void **)&dy, sizeof(double) * ne));
+    FHIPX(hipMemcpyAsync(dx, x, sizeof(double) * ne, hipMemcpyHostToDevice, f->st));
+    int rc = filter_apply_dev(f, dx, trans, dy, f->st);
+    if (!rc) FHIPX(hipMemcpyAsync(y, dy, sizeof(double) * ne, hipMemcpyDeviceToHost, f->st));
+    FHIPX(hipStreamSynchronize(f->st));
+    dfree(dx); dfree(dy);
+    return rc;
+}
+
+int tmg_filter_rowsum(const tmg_filter *f, double *w) {
+    if (!f || !w) return TMG_ERR_ARG;
+    const int64_t ne = (int64_t)f->nelx * f->nely * f->nelz;
+    if (hipMemcpy(w, f->w, sizeof(double) * ne, hipMemcpyDeviceToHost) != hipSuccess) return TMG_ERR_HIP;
+    return TMG_OK;
+}
+
+void tmg_filter_destroy(tmg_filter *f) {
+    if (!f) return;
+    filter_free(f);
+    delete f;
+}
+
+int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens, const tmg_filter *filt,
+             double *Fgrad) {
+    if (!h || !u || !x || !Fgrad) return TMG_ERR_ARG;
+    const DevLevel &l0 = h->lev[0];
+    if (filt && (filt->nelx != l0.nelx || filt->nely != l0.nely || filt->nelz != l0.nelz))
+        return set_err(h, TMG_ERR_ARG, "filter grid does not match the problem grid");
+    double *du = nullptr, *dxe = nullptr, *dg = nullptr, *dF = nullptr;
+    uint8_t *dfd = nullptr;
+    TRY(dalloc(h, &du, l0.ndof));
+    TRY(dalloc(h, &dxe, l0.ne));
+    TRY(dalloc(h, &dg, l0.ne));
+    TRY(dalloc(h, &dF, l0.ne));
+    if (freedens) TRY(dalloc(h, &dfd, l0.ne));
+    HIPX(hipMemcpyAsync(du, u, sizeof(double) * l0.ndof, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(dxe, x, sizeof(double) * l0.ne, hipMemcpyHostToDevice, h->st));
+    if (freedens) HIPX(hipMemcpyAsync(dfd, freedens, l0.ne, hipMemcpyHostToDevice, h->st));
+    hipLaunchKernelGGL(k_grad0, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
+                       h->K0_64, h->penal, h->E0, h->emin, dg);
+    int rc = 0;
+    if (filt) rc = filter_apply_dev(const_cast<tmg_filter *>(filt), dg, 1, dF, h->st);
+    if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * l0.ne, hipMemcpyDeviceToHost, h->st));
+    HIPX(hipStreamSynchronize(h->st));
+    dfree(du); dfree(dxe); dfree(dg); dfree(dF); dfree(dfd);
+    return rc ? set_err(h, rc, "grad kernel launch failed") : TMG_OK;
+}
+
+}  // extern "C"
