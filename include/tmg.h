@@ -158,6 +158,39 @@ void tmg_filter_destroy(tmg_filter *f);
  * (preconditioner application). Requires a prior tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
+/* ---------------------------------------------------------------------------
+ * Serving (no counterpart in the reference, which solves one system at a time
+ * inside its SLP loop): a continuous-batching server for streams of
+ * equilibrium solves of uneven grid sizes. Requests are admitted FIFO as
+ * slots and device memory (estimated per request, against hbm_budget_bytes)
+ * free up; each admitted request gets its own device-resident hierarchy and
+ * stream, and the PCG iterations of all running requests overlap on the
+ * device. Each result is bitwise identical to the same solve run alone.
+ * ------------------------------------------------------------------------- */
+typedef struct tmg_server tmg_server;
+
+int tmg_server_create(double hbm_budget_bytes, int max_active, tmg_server **out);
+/* Queue one solve: prob/mg/precision as tmg_setup, x the filtered densities
+ * (n_el), f the load (nfree), x0 the initial guess (nfree, NULL = 0),
+ * t_arrival the arrival time on the server clock (< 0 = now). Inputs are
+ * copied. Returns TMG_ERR_BUDGET if the request can never fit the budget. */
+int tmg_server_submit(tmg_server *s, const tmg_problem *prob, const tmg_mg_params *mg, int precision,
+                      const double *x, const double *f, const double *x0, double t_arrival, int64_t *id);
+/* Wait for a request (timeout_s < 0: no limit). Returns 0 when it has
+ * finished (state 3) or failed (state < 0), 1 on timeout. */
+int tmg_server_wait(tmg_server *s, int64_t id, double timeout_s, int *state);
+/* Solution (nfree), reference iteration count, final relative residual and
+ * times[4] = {arrival, admission, start of PCG, completion} on the server clock. */
+int tmg_server_result(tmg_server *s, int64_t id, double *u, int *it, double *relres, double *times);
+int tmg_server_release(tmg_server *s, int64_t id);
+/* stats[6] = {bytes reserved, peak bytes reserved, queued, running, done, clock}. */
+int tmg_server_stats(tmg_server *s, double *stats);
+double tmg_server_clock(const tmg_server *s);
+/* Device bytes the server reserves for a request of this shape. */
+int64_t tmg_server_estimate(const tmg_problem *prob, const tmg_mg_params *mg, int precision);
+const char *tmg_server_last_error(const tmg_server *s);
+void tmg_server_destroy(tmg_server *s);
+
 /* Message of the last failed call on h (or of the last failed tmg_setup when h is NULL). */
 const char *tmg_last_error(const tmg_solver *h);
 

@@ -69,3 +69,14 @@ def test_null_handle_calls_fail_cleanly():
     assert L.tmg_update_density(None, None) == _lib.TMG_ERR_ARG
     assert L.tmg_ngrids(None) == 0
     L.tmg_destroy(None)
+
+
+def test_server_rejects_bad_arguments_and_estimates():
+    h = C.c_void_p()
+    assert tmg.lib().tmg_server_create(0.0, 4, C.byref(h)) == _lib.TMG_ERR_ARG
+    assert tmg.lib().tmg_server_create(1e9, 0, C.byref(h)) == _lib.TMG_ERR_ARG
+    from tmg import serve
+    small = serve.estimate_bytes(P.cantilever(32, 16, 16), 5)
+    large = serve.estimate_bytes(P.cantilever(128, 64, 64), 5)
+    assert 0 < small < large < 2e9
+    assert 0 < serve.estimate_bytes(P.cantilever(128, 64, 64), 5, "fp64") < 2e9

@@ -1,0 +1,105 @@
+"""Serving: continuous batching of solves of uneven sizes through the C-ABI
+server (tmg_server_*): each result is the same alone and in the mix, memory
+admission respects the budget, failures are reported per request."""
+import sys
+
+import numpy as np
+import pytest
+
+from tmg import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _requests(n=8, seed=7):
+    from tmg import serve
+    return serve.stream_workload(n=n, seed=seed, sizes=(32, 48, 64))
+
+
+def test_same_alone_and_in_the_mix():
+    from tmg import serve
+    reqs = _requests(8)
+    _say("alone")
+    alone = []
+    with_srv = serve.Server(budget_bytes=8e9, max_active=1)
+    for r in reqs:
+        rid = with_srv.submit(r["prob"], r["x"], r["ngrids"])
+        assert with_srv.wait(rid, 120)[0]
+        alone.append(with_srv.result(rid))
+    with_srv.close()
+    _say("mix")
+    srv = serve.Server(budget_bytes=8e9, max_active=8)
+    ids = [srv.submit(r["prob"], r["x"], r["ngrids"]) for r in reqs]
+    mix = []
+    for rid in ids:
+        assert srv.wait(rid, 120)[0]
+        mix.append(srv.result(rid))
+    st = srv.stats()
+    srv.close()
+    for a, m, r in zip(alone, mix, reqs):
+        assert a["rc"] == 0 and m["rc"] == 0 and a["relres"] <= 1e-8
+        assert a["it"] == m["it"]
+        assert np.array_equal(a["u"], m["u"]), r["i"]
+    assert st["peak_bytes_reserved"] <= 8e9
+    # overlap: in the mix, some request started before another one finished
+    starts = sorted(m["t_start"] for m in mix)
+    dones = sorted(m["t_done"] for m in mix)
+    assert starts[1] < dones[0]
+
+
+def test_server_matches_direct_solver():
+    import tmg
+    from tmg import serve
+    r = _requests(1, seed=9)[0]
+    s = tmg.Solver(r["prob"], r["ngrids"], precision="mixed")
+    s.update_density(r["x"])
+    u, it, rr, rc = s.solve(r["prob"].f)
+    srv = serve.Server(budget_bytes=4e9, max_active=2)
+    rid = srv.submit(r["prob"], r["x"], r["ngrids"])
+    srv.wait(rid)
+    res = srv.result(rid)
+    srv.close()
+    assert res["it"] == it and np.array_equal(res["u"], u)
+
+
+def test_budget_admission_and_rejection():
+    from tmg import TmgError, serve
+    reqs = _requests(4, seed=11)
+    big = max(serve.estimate_bytes(r["prob"], r["ngrids"]) for r in reqs)
+    srv = serve.Server(budget_bytes=big * 1.01, max_active=4)   # one request at a time fits
+    ids = [srv.submit(r["prob"], r["x"], r["ngrids"]) for r in reqs]
+    res = []
+    for rid in ids:
+        assert srv.wait(rid, 120)[0]
+        res.append(srv.result(rid))
+    st = srv.stats()
+    assert all(r["rc"] == 0 for r in res)
+    assert st["peak_bytes_reserved"] <= big * 1.01
+    # admissions were serialised by the budget
+    order = sorted(res, key=lambda r: r["t_admit"])
+    for a, b in zip(order, order[1:]):
+        assert b["t_admit"] >= a["t_done"] - 1e-3
+    huge = P.cantilever(128, 64, 64)
+    with pytest.raises(TmgError) as e:
+        srv.submit(huge, np.full(huge.nelem, 0.5), 5)
+    assert e.value.code == -7
+    srv.close()
+
+
+def test_failed_request_is_reported():
+    from tmg import TmgError, serve
+    r = _requests(1, seed=12)[0]
+    srv = serve.Server(budget_bytes=4e9, max_active=2)
+    rid = srv.submit(r["prob"], -np.ones_like(r["x"]), r["ngrids"])
+    ok, state = srv.wait(rid, 60)
+    assert ok and state < 0
+    with pytest.raises(TmgError):
+        srv.result(rid)
+    rid2 = srv.submit(r["prob"], r["x"], r["ngrids"])   # the server keeps serving
+    assert srv.wait(rid2, 60)[0] and srv.result(rid2)["rc"] == 0
+    srv.close()

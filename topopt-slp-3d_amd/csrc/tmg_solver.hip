@@ -9,6 +9,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@ using namespace tmg;
 namespace {
 
 constexpr int MAXL = 12;
+constexpr int GRAPH_ITERS = 4;
 thread_local std::string g_setup_err;
 
 struct DevLevel {
@@ -64,7 +66,19 @@ struct tmg_solver {
     PcgScalars *d_sc = nullptr;
     PcgScalars *h_sc = nullptr;
     int zchunk = 4;
+    // PCG iteration captured once as a HIP graph (all pointers are fixed for
+    // the handle's lifetime), replayed GRAPH_ITERS iterations per launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = true;
     hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
+    // externally owned resources (a server slot): device memory comes from a
+    // stream-ordered pool, and the stream, rocBLAS handle and pinned scalar
+    // block are lent, so nothing here synchronises the whole device
+    hipMemPool_t pool = nullptr;
+    bool own_stream = true, own_rb = true, own_hsc = true;
+    size_t bytes = 0;                 // device bytes held by this handle
     bool have_op = false;
     std::string err;
 };
@@ -88,19 +102,27 @@ namespace {
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 template <typename T>
 int dalloc(tmg_solver *h, T **p, int64_t n) {
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)std::max<int64_t>(n, 1));
+    const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    hipError_t e = h->pool ? hipMallocFromPoolAsync((void **)p, bytes, h->pool, h->st) : hipMalloc((void **)p, bytes);
     if (e != hipSuccess) {
-        h->err = "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) + " bytes failed: " + hipGetErrorString(e);
+        *p = nullptr;
+        h->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
         return TMG_ERR_OOM;
     }
+    h->bytes += round256(bytes);
     return 0;
 }
 
 template <typename T>
-void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
+void dfree(tmg_solver *h, T *&p) {
+    if (p) {
+        if (h->pool) (void)hipFreeAsync((void *)p, h->st);
+        else (void)hipFree((void *)p);
+    }
     p = nullptr;
 }
 
@@ -235,8 +257,9 @@ int check_launch(tmg_solver *h) {
     return 0;
 }
 
-// Device-resident solve on full-length FP64 vectors px (x0 in, x out) and pf.
-int solve_resident(tmg_solver *h, int *it, double *relres) {
+// Start a solve on the handle's px (x0) and pf: r = f - A x0, norms, and the
+// PCG iteration graph (captured once per handle).
+int solve_begin(tmg_solver *h) {
     if (!h->have_op) {
         h->err = "no operator: call tmg_update_density first";
         return TMG_ERR_STATE;
@@ -257,15 +280,45 @@ int solve_resident(tmg_solver *h, int *it, double *relres) {
     hipLaunchKernelGGL(k_finalize<FIN_INIT>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, (int)gdot,
                        h->partial2, h->d_sc);
     TRY(check_launch(h));
-    int chunk = 4;
-    for (;;) {
-        for (int k = 0; k < chunk; ++k) pcg_iteration(h);
-        TRY(check_launch(h));
-        HIPX(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->st));
-        HIPX(hipStreamSynchronize(h->st));
-        if (h->h_sc->done) break;
-        chunk = std::min(chunk * 2, 32);
+    if (h->use_graph && !h->graph_exec) {
+        // capture GRAPH_ITERS consecutive iterations; iterations past
+        // convergence are skipped on the device through the done flag
+        HIPX(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < GRAPH_ITERS; ++k) pcg_iteration(h);
+        hipGraph_t g = nullptr;
+        HIPX(hipStreamEndCapture(h->st, &g));
+        h->graph = g;
+        HIPX(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
     }
+    if (!h->ev) HIPX(hipEventCreateWithFlags(&h->ev, hipEventDisableTiming));
+    return 0;
+}
+
+// Launch nchunk x GRAPH_ITERS iterations, then an asynchronous copy of the
+// scalars to pinned memory and the completion event.
+int solve_launch(tmg_solver *h, int nchunk) {
+    if (h->graph_exec) {
+        for (int k = 0; k < nchunk; ++k) HIPX(hipGraphLaunch(h->graph_exec, h->st));
+    } else {
+        for (int k = 0; k < nchunk * GRAPH_ITERS; ++k) pcg_iteration(h);
+    }
+    TRY(check_launch(h));
+    HIPX(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->st));
+    HIPX(hipEventRecord(h->ev, h->st));
+    return 0;
+}
+
+// 1 when the last launched chunk has completed (h_sc is then current), 0 if
+// it is still running, < 0 on error.
+int solve_poll(tmg_solver *h) {
+    hipError_t e = hipEventQuery(h->ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    h->err = std::string("hipEventQuery: ") + hipGetErrorString(e);
+    return TMG_ERR_HIP;
+}
+
+int solve_result(tmg_solver *h, int *it, double *relres) {
     *it = h->h_sc->it;
     *relres = h->h_sc->nr;
     if (!std::isfinite(h->h_sc->nr)) {
@@ -273,6 +326,19 @@ int solve_resident(tmg_solver *h, int *it, double *relres) {
         return TMG_ERR_NOT_SPD;
     }
     return (h->h_sc->nr > h->mg.tolMG) ? TMG_NOT_CONVERGED : TMG_OK;
+}
+
+// Blocking device-resident solve on px (x0 in, x out) and pf.
+int solve_resident(tmg_solver *h, int *it, double *relres) {
+    TRY(solve_begin(h));
+    int chunk = 1;
+    for (;;) {
+        TRY(solve_launch(h, chunk));
+        HIPX(hipEventSynchronize(h->ev));
+        if (h->h_sc->done) break;
+        chunk = std::min(chunk * 2, 8);
+    }
+    return solve_result(h, it, relres);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,9 +358,11 @@ int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
     TRY(dalloc(h, &lv.mask, lv.ndof));
     TRY(dalloc(h, &lv.d_free, lv.nfree));
     TRY(dalloc(h, &lv.d_cidx, lv.ndof));
-    HIPX(hipMemcpy(lv.mask, mask.data(), lv.ndof, hipMemcpyHostToDevice));
-    if (lv.nfree) HIPX(hipMemcpy(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice));
-    HIPX(hipMemcpy(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice));
+    HIPX(hipMemcpyAsync(lv.mask, mask.data(), lv.ndof, hipMemcpyHostToDevice, h->st));
+    if (lv.nfree)
+        HIPX(hipMemcpyAsync(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice, h->st));
+    HIPX(hipStreamSynchronize(h->st));  // host staging vectors go out of scope
     const size_t vs = h->prec == TMG_MIXED ? sizeof(float) : sizeof(double);
     TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * vs)));
     TRY(dalloc(h, (char **)&lv.x, (int64_t)(lv.ndof * vs)));
@@ -313,19 +381,26 @@ void destroy_all(tmg_solver *h) {
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (int l = 0; l < MAXL; ++l) {
         DevLevel &lv = h->lev[l];
-        dfree(lv.mask); dfree(lv.S64); dfree(lv.S32); dfree(lv.inv64); dfree(lv.inv32);
-        dfree(lv.b); dfree(lv.x); dfree(lv.t); dfree(lv.d_free); dfree(lv.d_cidx);
+        dfree(h, lv.mask); dfree(h, lv.S64); dfree(h, lv.S32); dfree(h, lv.inv64); dfree(h, lv.inv32);
+        dfree(h, lv.b); dfree(h, lv.x); dfree(h, lv.t); dfree(h, lv.d_free); dfree(h, lv.d_cidx);
     }
-    dfree(h->K0_64); dfree(h->K0_32); dfree(h->E64); dfree(h->E32); dfree(h->xdens);
-    dfree(h->Ahat[0]); dfree(h->Ahat[1]);
-    dfree(h->Ainv); dfree(h->cb); dfree(h->cx); dfree(h->d_info);
-    dfree(h->px); dfree(h->pr); dfree(h->pd); dfree(h->pq); dfree(h->pf);
-    dfree(h->partial); dfree(h->partial2); dfree(h->d_sc);
-    if (h->h_sc) (void)hipHostFree(h->h_sc);
+    dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
+    dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
+    dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
+    dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pq); dfree(h, h->pf);
+    dfree(h, h->partial); dfree(h, h->partial2); dfree(h, h->d_sc);
+    if (h->h_sc && h->own_hsc) (void)hipHostFree(h->h_sc);
     h->h_sc = nullptr;
-    if (h->rb) rocblas_destroy_handle(h->rb);
+    if (h->ev) (void)hipEventDestroy(h->ev);
+    h->ev = nullptr;
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    if (h->rb && h->own_rb) rocblas_destroy_handle(h->rb);
     h->rb = nullptr;
-    if (h->st) (void)hipStreamDestroy(h->st);
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    if (h->st && h->own_stream) (void)hipStreamDestroy(h->st);
     h->st = nullptr;
 }
 
@@ -394,6 +469,7 @@ int refresh(tmg_solver *h) {
                        lc.d_cidx, nc, h->Ainv);
     TRY(check_launch(h));
     if (nc > 0) {
+        rocblas_set_stream(h->rb, h->st);  // the handle may be shared between solvers
         rocblas_status s1 = rocsolver_dpotrf(h->rb, rocblas_fill_upper, (rocblas_int)nc, h->Ainv, (rocblas_int)nc, h->d_info);
         int info = 0;
         HIPX(hipMemcpyAsync(&info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -428,7 +504,7 @@ int upload_free(tmg_solver *h, int l, const double *hv, double *d_full) {
     hipLaunchKernelGGL((k_scatter_full<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, lv.d_cidx, tmp,
                        d_full, nullptr);
     HIPX(hipStreamSynchronize(h->st));
-    dfree(tmp);
+    dfree(h, tmp);
     return 0;
 }
 
@@ -440,7 +516,7 @@ int download_free(tmg_solver *h, int l, const double *d_full, double *hv) {
                        nullptr);
     HIPX(hipMemcpyAsync(hv, tmp, sizeof(double) * lv.nfree, hipMemcpyDeviceToHost, h->st));
     HIPX(hipStreamSynchronize(h->st));
-    dfree(tmp);
+    dfree(h, tmp);
     return 0;
 }
 
@@ -454,9 +530,18 @@ int set_err(tmg_solver *h, int rc, const std::string &m) {
 // ===========================================================================
 // C-ABI
 // ===========================================================================
-extern "C" {
+namespace {
 
-int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, tmg_solver **out) {
+// Resources lent by a caller that runs many handles (the server): a stream,
+// a rocBLAS handle, a pinned scalar block and a stream-ordered memory pool.
+struct ExtRes {
+    hipStream_t st = nullptr;
+    rocblas_handle rb = nullptr;
+    PcgScalars *h_sc = nullptr;
+    hipMemPool_t pool = nullptr;
+};
+
+int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, const ExtRes *ext, tmg_solver **out) {
     g_setup_err.clear();
     if (!prob || !mg || !out) { g_setup_err = "null argument"; return TMG_ERR_ARG; }
     *out = nullptr;
@@ -489,6 +574,14 @@ int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, t
     h->mg = *mg;
     h->prec = precision;
     h->L = mg->ngrids;
+    if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
+    if (ext) {
+        h->st = ext->st;
+        h->rb = ext->rb;
+        h->h_sc = ext->h_sc;
+        h->pool = ext->pool;
+        h->own_stream = h->own_rb = h->own_hsc = false;
+    }
     auto fail = [&](int rc) {
         g_setup_err = h->err;
         destroy_all(h);
@@ -514,7 +607,7 @@ int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, t
             mask[fd - 1] = 1;
         }
     }
-    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    if (!h->st && hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
         h->err = "hipStreamCreate failed";
         return fail(TMG_ERR_HIP);
     }
@@ -561,22 +654,33 @@ int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, t
     if ((rc = dalloc(h, &h->partial, h->npartial)) || (rc = dalloc(h, &h->partial2, h->npartial)) ||
         (rc = dalloc(h, &h->d_sc, 1)))
         return fail(rc);
-    if (hipHostMalloc((void **)&h->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) {
+    if (!h->h_sc && hipHostMalloc((void **)&h->h_sc, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) {
         h->err = "hipHostMalloc failed";
         return fail(TMG_ERR_OOM);
     }
-    if (hipMemcpy(h->K0_64, h->ke, sizeof(h->ke), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->K0_32, ke32, sizeof(ke32), hipMemcpyHostToDevice) != hipSuccess) {
-        h->err = "hipMemcpy of Ke failed";
+    if (hipMemcpyAsync(h->K0_64, h->ke, sizeof(h->ke), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->K0_32, ke32, sizeof(ke32), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
+        h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
     }
-    if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
-        h->err = "rocblas_create_handle failed";
-        return fail(TMG_ERR_HIP);
+    if (!h->rb) {
+        if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
+            h->err = "rocblas_create_handle failed";
+            return fail(TMG_ERR_HIP);
+        }
+        rocblas_set_stream(h->rb, h->st);
     }
-    rocblas_set_stream(h->rb, h->st);
     *out = h;
     return TMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, tmg_solver **out) {
+    return setup_impl(prob, mg, precision, nullptr, out);
 }
 
 int tmg_update_density_device(tmg_solver *h, const double *d_x) {
@@ -658,7 +762,7 @@ int tmg_apply(tmg_solver *h, int level, const double *x, double *y) {
         rc = check_launch(h);
     }
     if (!rc) rc = download_free(h, l, dy, y);
-    dfree(dx); dfree(dy);
+    dfree(h, dx); dfree(h, dy);
     return rc;
 }
 
@@ -681,7 +785,7 @@ int tmg_smooth(tmg_solver *h, int level, const double *b, const double *x0, int 
         rc = check_launch(h);
     }
     if (!rc) rc = download_free(h, l, xa, x);
-    dfree(db); dfree(xa); dfree(xb);
+    dfree(h, db); dfree(h, xa); dfree(h, xb);
     return rc;
 }
 
@@ -699,7 +803,7 @@ int tmg_prolong(tmg_solver *h, int level, const double *xc, double *xf) {
         rc = check_launch(h);
     }
     if (!rc) rc = download_free(h, level - 1, df, xf);
-    dfree(dc); dfree(df);
+    dfree(h, dc); dfree(h, df);
     return rc;
 }
 
@@ -717,7 +821,7 @@ int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc) {
         rc = check_launch(h);
     }
     if (!rc) rc = download_free(h, level, dc, xc);
-    dfree(dc); dfree(df);
+    dfree(h, dc); dfree(h, df);
     return rc;
 }
 
@@ -810,9 +914,15 @@ struct tmg_filter {
 
 namespace {
 
+template <typename T>
+void hfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
 void filter_free(tmg_filter *f) {
     if (f->st) (void)hipStreamSynchronize(f->st);
-    dfree(f->off); dfree(f->wt); dfree(f->fixed); dfree(f->w);
+    hfree(f->off); hfree(f->wt); hfree(f->fixed); hfree(f->w);
     if (f->st) (void)hipStreamDestroy(f->st);
     f->st = nullptr;
 }
@@ -925,7 +1035,7 @@ int tmg_filter_apply(tmg_filter *f, const double *x, int trans, double *y) {
     int rc = filter_apply_dev(f, dx, trans, dy, f->st);
     if (!rc) FHIPX(hipMemcpyAsync(y, dy, sizeof(double) * ne, hipMemcpyDeviceToHost, f->st));
     FHIPX(hipStreamSynchronize(f->st));
-    dfree(dx); dfree(dy);
+    hfree(dx); hfree(dy);
     return rc;
 }
 
@@ -964,8 +1074,13 @@ int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *fre
     if (filt) rc = filter_apply_dev(const_cast<tmg_filter *>(filt), dg, 1, dF, h->st);
     if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * l0.ne, hipMemcpyDeviceToHost, h->st));
     HIPX(hipStreamSynchronize(h->st));
-    dfree(du); dfree(dxe); dfree(dg); dfree(dF); dfree(dfd);
+    dfree(h, du); dfree(h, dxe); dfree(h, dg); dfree(h, dF); dfree(h, dfd);
     return rc ? set_err(h, rc, "grad kernel launch failed") : TMG_OK;
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Serving: continuous batching of many solves
+// ===========================================================================
+#include "tmg_server.inc.hpp"

@@ -79,6 +79,16 @@ _SIGS = {
     "tmg_filter_rowsum": (_i, [_p, _pd]),
     "tmg_filter_destroy": (None, [_p]),
     "tmg_time_kernel": (_i, [_p, _i, _i, _pd]),
+    "tmg_server_create": (_i, [_d, _i, C.POINTER(_p)]),
+    "tmg_server_submit": (_i, [_p, C.POINTER(Problem_t), C.POINTER(MgParams_t), _i, _pd, _pd, _pd, _d, _pi64]),
+    "tmg_server_wait": (_i, [_p, C.c_int64, _d, _pi]),
+    "tmg_server_result": (_i, [_p, C.c_int64, _pd, _pi, _pd, _pd]),
+    "tmg_server_release": (_i, [_p, C.c_int64]),
+    "tmg_server_stats": (_i, [_p, _pd]),
+    "tmg_server_clock": (_d, [_p]),
+    "tmg_server_estimate": (C.c_int64, [C.POINTER(Problem_t), C.POINTER(MgParams_t), _i]),
+    "tmg_server_last_error": (C.c_char_p, [_p]),
+    "tmg_server_destroy": (None, [_p]),
     "tmg_last_error": (C.c_char_p, [_p]),
     "tmg_destroy": (None, [_p]),
 }
