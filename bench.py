@@ -194,8 +194,8 @@ def main():
             "workload": f"cantilever {nelx}x{nely}x{nelz} Q1, {prob.nfree} free DOFs, {L}-level GMG V(1,1) "
                         f"Jacobi omega 0.6, filtered random densities seed {args.seed}, tol 1e-8",
             "n_free_dofs": prob.nfree,
-            "fp64": {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in fp64.items()},
-            "mixed": {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in mixed.items()},
+            "fp64": {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in fp64.items()},
+            "mixed": {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in mixed.items()},
             "speedup_mixed_vs_fp64_time_to_solution": round(fp64["t_solve"] / mixed["t_solve"], 3),
             "kernels_ms": k,
         },

@@ -195,14 +195,52 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int z
 }
 
 // ---------------------------------------------------------------------------
-// Coarse-level operator: assembled 27-point block stencil (Galerkin P'AP).
+// Coarse-level operator: assembled 27-point block stencil of the Galerkin
+// operator P'AP, stored symmetrically: only the centre block and the 13
+// "forward" neighbour blocks (offset index k = (dz+1)*9 + (dx+1)*3 + (dy+1)
+// > 13) are kept, 14 blocks x 9 = 126 coefficients per node; the block of a
+// backward neighbour j is the transpose of j's forward block towards i.
 // Layout is structure-of-arrays blocked by wavefront: coefficient
-// (k*9 + c*3 + d) of node i sits at S[(i/64)*243*64 + (k*9 + c*3 + d)*64 + i%64],
-// k = neighbour offset (dz+1)*9 + (dx+1)*3 + (dy+1), so a wave's loads of one
-// coefficient are one contiguous 64-element run and every offset is a
-// compile-time constant.
+// f*9 + c*3 + d (f = k - 13) of node i sits at
+// S[(i/64)*126*64 + (f*9 + c*3 + d)*64 + i%64], so a wave's loads of one
+// coefficient are one contiguous 64-element run.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline int64_t sidx(int64_t i, int coef) { return (i >> 6) * (243 * 64) + coef * 64 + (i & 63); }
+constexpr int NCOEF = 126;
+__host__ __device__ inline int64_t sidx(int64_t i, int coef) { return (i >> 6) * (NCOEF * 64) + coef * 64 + (i & 63); }
+
+// y += contributions of the 9 neighbours in plane dz = DZ.
+template <typename T, int DZ>
+__device__ inline void stencil_plane(int64_t i, int ix, int iy, int iz, int nx, int ny, int nz, int64_t plane,
+                                     const T *__restrict__ x, const T *__restrict__ S, T (&y)[3]) {
+    const int jz = iz + DZ;
+    const bool vz = jz >= 0 && jz < nz;
+    const int64_t jzb = (int64_t)min(max(jz, 0), nz - 1) * plane;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int dx = q / 3 - 1, dy = q % 3 - 1;
+        const int k = (DZ + 1) * 9 + q;
+        const int jx = ix + dx, jy = iy + dy;
+        const bool v = vz && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+        const int64_t j = jzb + (int64_t)min(max(jx, 0), nx - 1) * ny + min(max(jy, 0), ny - 1);
+        const T vf = v ? T(1) : T(0);   // a clamped neighbour contributes nothing
+        T xv[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xv[d] = x[3 * j + d] * vf;
+        if (k >= 13) {
+            const T *Sw = S + sidx(i, (k - 13) * 9);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) y[c] += Sw[(c * 3 + d) * 64] * xv[d];
+        } else {
+            const T *Sw = S + sidx(j, (13 - k) * 9);   // (A_ji)^T
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) y[c] += Sw[(d * 3 + c) * 64] * xv[d];
+        }
+    }
+}
 
 template <typename T, int EPI>
 __global__ __launch_bounds__(TPB) void k_stencil(int nx, int ny, int nz, const T *__restrict__ x,
@@ -213,37 +251,26 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx, int ny, int nz, const T
     if (done && *done) return;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t nn = plane * nz;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    const bool act = i < nn;
+    const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const bool act = i0 < nn;
     double dacc = 0.0;
     if (act) {
-        const int iz = (int)(i / plane);
-        const int ix = (int)((i / ny) % nx);
-        const int iy = (int)(i % ny);
+        int64_t i = i0;
+        int iz = (int)(i / plane);
+        int ix = (int)((i / ny) % nx);
+        int iy = (int)(i % ny);
         T y[3] = {T(0), T(0), T(0)};
-        const T *Sw = S + (i >> 6) * (243 * 64) + (i & 63);
-#pragma unroll 1
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int64_t jzb = (int64_t)min(max(iz + dz, 0), nz - 1) * plane;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                const int dx = q / 3 - 1, dy = q % 3 - 1;
-                const int k = (dz + 1) * 9 + q;
-                // clamped neighbour: its stencil coefficients are exactly zero
-                const int jx = min(max(ix + dx, 0), nx - 1), jy = min(max(iy + dy, 0), ny - 1);
-                const int64_t j = jzb + (int64_t)jx * ny + jy;
-                T xv[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) xv[d] = x[3 * j + d];
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) y[c] += Sw[(k * 9 + c * 3 + d) * 64] * xv[d];
-            }
-        }
+        // one plane of neighbours at a time: opaque copies of the coordinates
+        // keep the next plane's loads from being hoisted (and held in
+        // registers) before this plane's products are done
+        stencil_plane<T, -1>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
+        asm volatile("" : "+v"(i), "+v"(ix), "+v"(iy), "+v"(iz) : "v"(y[0]), "v"(y[1]), "v"(y[2]));
+        stencil_plane<T, 0>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
+        asm volatile("" : "+v"(i), "+v"(ix), "+v"(iy), "+v"(iz) : "v"(y[0]), "v"(y[1]), "v"(y[2]));
+        stencil_plane<T, 1>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int64_t dof = 3 * i + c;
+            const int64_t dof = 3 * i0 + c;
             const bool m = mask[dof] != 0;
             if (EPI == EPI_PLAIN) {
                 out[dof] = m ? y[c] : T(0);
@@ -512,7 +539,7 @@ __global__ __launch_bounds__(TPB) void k_diag_stencil(int64_t nn, const double *
     if (i >= nn) return;
     for (int c = 0; c < 3; ++c) {
         const int64_t dof = 3 * i + c;
-        const double dg = S[sidx(i, 13 * 9 + c * 3 + c)];
+        const double dg = S[sidx(i, c * 3 + c)];
         const double v = mask[dof] ? 1.0 / ((1.0 / omega) * dg) : 0.0;
         inv64[dof] = v;
         if (inv32) inv32[dof] = (float)v;
@@ -618,7 +645,8 @@ __global__ __launch_bounds__(TPB) void k_galerkin_coarse(int cnelx, int cnely, i
 }
 
 // Assembled block stencil of a level from its element matrices, masked by the
-// level's free-DOF mask on both sides. One thread per (node, neighbour).
+// level's free-DOF mask on both sides: the centre and forward blocks only
+// (k = 13..26). One thread per (node, forward neighbour).
 template <typename T32>
 __global__ __launch_bounds__(TPB) void k_stencil_assemble(int nelx, int nely, int nelz, const double *__restrict__ Ahat,
                                                           const uint8_t *__restrict__ mask, double *__restrict__ S64,
@@ -626,9 +654,10 @@ __global__ __launch_bounds__(TPB) void k_stencil_assemble(int nelx, int nely, in
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t nn = (int64_t)nx * ny * nz, ne = (int64_t)nelx * nely * nelz;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= nn * 27) return;
-    const int k = (int)(t / nn);
-    const int64_t i = t - (int64_t)k * nn;
+    if (t >= nn * 14) return;
+    const int f = (int)(t / nn);
+    const int k = 13 + f;
+    const int64_t i = t - (int64_t)f * nn;
     const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
     const int jz = iz + k / 9 - 1, jx = ix + (k / 3) % 3 - 1, jy = iy + k % 3 - 1;
     double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -651,21 +680,23 @@ __global__ __launch_bounds__(TPB) void k_stencil_assemble(int nelx, int nely, in
     }
     for (int c = 0; c < 3; ++c)
         for (int d = 0; d < 3; ++d) {
-            const int64_t o = sidx(i, k * 9 + c * 3 + d);
+            const int64_t o = sidx(i, f * 9 + c * 3 + d);
             S64[o] = acc[c][d];
             if (S32) S32[o] = (T32)acc[c][d];
         }
 }
 
-// Dense coarsest operator (free DOFs, row-major) from the coarsest stencil.
+// Dense coarsest operator (free DOFs, row-major) from the coarsest stencil:
+// each forward block fills its entries and, mirrored, the transposed ones.
 __global__ __launch_bounds__(TPB) void k_dense_from_stencil(int nx, int ny, int nz, const double *__restrict__ S,
                                                             const int64_t *__restrict__ cidx, int64_t n,
                                                             double *__restrict__ Ad) {
     const int64_t nn = (int64_t)nx * ny * nz;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= nn * 27) return;
-    const int k = (int)(t / nn);
-    const int64_t i = t - (int64_t)k * nn;
+    if (t >= nn * 14) return;
+    const int f = (int)(t / nn);
+    const int k = 13 + f;
+    const int64_t i = t - (int64_t)f * nn;
     const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
     const int jz = iz + k / 9 - 1, jx = ix + (k / 3) % 3 - 1, jy = iy + k % 3 - 1;
     if (jz < 0 || jz >= nz || jx < 0 || jx >= nx || jy < 0 || jy >= ny) return;
@@ -676,7 +707,9 @@ __global__ __launch_bounds__(TPB) void k_dense_from_stencil(int nx, int ny, int 
         for (int d = 0; d < 3; ++d) {
             const int64_t col = cidx[3 * j + d];
             if (col < 0) continue;
-            Ad[row * n + col] = S[sidx(i, k * 9 + c * 3 + d)];
+            const double v = S[sidx(i, f * 9 + c * 3 + d)];
+            Ad[row * n + col] = v;
+            if (f > 0) Ad[col * n + row] = v;
         }
     }
 }

@@ -59,7 +59,7 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
         add(8 * ndof);                    // inv64
         if (prec == TMG_MIXED) add(4 * ndof);
         if (l > 0) {
-            const size_t ns = (size_t)((nn + 63) / 64) * 64 * 243;
+            const size_t ns = (size_t)((nn + 63) / 64) * 64 * NCOEF;
             add(8 * ns);
             if (prec == TMG_MIXED) add(4 * ns);
         }

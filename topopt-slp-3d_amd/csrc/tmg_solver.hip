@@ -370,7 +370,7 @@ int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
     TRY(dalloc(h, &lv.inv64, lv.ndof));
     if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.inv32, lv.ndof));
     if (l > 0) {
-        const int64_t ns = ((lv.nn + 63) / 64) * 64 * 243;  // wave-blocked stencil storage
+        const int64_t ns = ((lv.nn + 63) / 64) * 64 * NCOEF;  // wave-blocked symmetric stencil storage
         TRY(dalloc(h, &lv.S64, ns));
         if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.S32, ns));
     }
@@ -456,7 +456,7 @@ int refresh(tmg_solver *h) {
             hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(lv.ne * 64)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
                                lv.nelz, h->Ahat[l & 1], h->lev[l - 1].mask, dst);
         }
-        hipLaunchKernelGGL((k_stencil_assemble<float>), dim3(nblk(lv.nn * 27)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
+        hipLaunchKernelGGL((k_stencil_assemble<float>), dim3(nblk(lv.nn * 14)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
                            lv.nelz, dst, lv.mask, lv.S64, h->prec == TMG_MIXED ? lv.S32 : nullptr);
         hipLaunchKernelGGL(k_diag_stencil, dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nn, lv.S64, lv.mask, om, lv.inv64,
                            h->prec == TMG_MIXED ? lv.inv32 : nullptr);
@@ -465,7 +465,7 @@ int refresh(tmg_solver *h) {
     DevLevel &lc = h->lev[h->L - 1];
     const int64_t nc = h->nc;
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc * nc, 0.0, h->Ainv);
-    hipLaunchKernelGGL(k_dense_from_stencil, dim3(nblk(lc.nn * 27)), dim3(TPB), 0, h->st, lc.nx, lc.ny, lc.nz, lc.S64,
+    hipLaunchKernelGGL(k_dense_from_stencil, dim3(nblk(lc.nn * 14)), dim3(TPB), 0, h->st, lc.nx, lc.ny, lc.nz, lc.S64,
                        lc.d_cidx, nc, h->Ainv);
     TRY(check_launch(h));
     if (nc > 0) {
