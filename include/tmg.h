@@ -155,7 +155,11 @@ void tmg_filter_destroy(tmg_filter *f);
  * buffers. which: 0 = fine-level operator apply in FP64 (the PCG's q = K d),
  * 1 = fine-level Jacobi sweep in the preconditioner precision, 2 = level-2
  * stencil Jacobi sweep in the preconditioner precision, 3 = one V-cycle
- * (preconditioner application). Requires a prior tmg_update_density. */
+ * (preconditioner application), 4 = fine-level residual with the first
+ * Jacobi sweep folded in, 5 = fine-level post-sweep with the prolongation
+ * folded in (both in the preconditioner precision), 6 = FP64 fine-level
+ * apply with the PCG direction update folded in. Requires a prior
+ * tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
 /* ---------------------------------------------------------------------------

@@ -50,19 +50,93 @@ __device__ inline void block_reduce_store(double v, double *partial, int64_t slo
 // chunk of z planes with a 3x3x3-node register window, so each x value is
 // loaded once per column step and every Ke entry is a wave-uniform scalar.
 // ---------------------------------------------------------------------------
-// Arguments: x input vector (full, fixed = 0); E element moduli; K0 the
-// 24x24 row-major element stiffness for E = 1; mask the free-DOF mask; b the
-// right-hand side (RESID, JACOBI); invD = omega / diag (0 at fixed DOFs,
-// JACOBI); out the result; partial per-block partial sums of x . (A x) (DOT);
-// done skips the launch once a PCG has converged. All pointers are
-// non-aliasing, which lets every Ke entry live in scalar registers.
-template <typename T, int EPI>
-__global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int zchunk, const T *__restrict__ x,
-                                             const T *__restrict__ E, const T *__restrict__ K0,
-                                             const uint8_t *__restrict__ mask, const T *__restrict__ b,
-                                             const T *__restrict__ invD, T *__restrict__ out,
-                                             double *__restrict__ partial, const int *__restrict__ done) {
-    if (done && *done) return;
+// Window sources: what the operator is applied to, computed while a node
+// plane is loaded.
+enum {
+    SRC_X = 0,     // x
+    SRC_JZ = 1,    // invD * r            (first Jacobi sweep from x0 = 0)
+    SRC_POST = 2,  // invD * r + P e_c    (that sweep plus the prolonged coarse correction)
+    SRC_DIR = 3    // beta d + s          (PCG direction update; d = s on the first iteration)
+};
+// Epilogues beyond PLAIN / RESID / JACOBI / DOT (r is the FP64 PCG residual):
+enum {
+    EPI_RESIDR = 4,  // out = r - A w
+    EPI_POSTR = 5,   // out = w + invD (r - A w);   partial r . out
+    EPI_DIRDOT = 6   // out = A w, dnew = w;         partial w . out
+};
+
+template <typename T, typename TS>
+struct EbeArgs {
+    int nelx, nely, nelz, zchunk;
+    const T *x;            // SRC_X input
+    const T *E;            // element moduli
+    const T *K0;           // 24x24 row-major element stiffness for E = 1
+    const uint8_t *mask;   // free-DOF mask
+    const T *b;            // right-hand side (RESID, JACOBI)
+    const T *invD;         // omega / diag, 0 at fixed DOFs
+    const double *r;       // PCG residual (SRC_JZ, SRC_POST, EPI_RESIDR, EPI_POSTR)
+    const T *ec;           // coarse correction (SRC_POST)
+    int cnx, cny;          // coarse node counts (SRC_POST)
+    const TS *s;           // preconditioned residual (SRC_DIR)
+    const double *dold;    // previous direction (SRC_DIR)
+    double *dnew;          // new direction (EPI_DIRDOT)
+    const PcgScalars *sc;  // beta, it (SRC_DIR)
+    T *out;
+    double *partial;       // per-block partial sums (DOT, POSTR, DIRDOT)
+    const int *done;       // skip once a PCG has converged
+};
+
+// Fine-level operator: element-by-element y = sum_e E_e Ke w_e, gathered per
+// node (no scatter). Each thread owns one (ix, iy) column of nodes and walks a
+// chunk of z planes with a 3x3x3-node register window; every Ke entry is a
+// wave-uniform scalar. TILED = true stages each new node plane of the block's
+// strip through LDS: the block's 256 consecutive plane indices p = ix*ny + iy
+// need the plane indices [p0 - ny - 1, p0 + 256 + ny], computed once per node
+// (source transform included) and loaded cooperatively, and the next plane's
+// loads are issued before the current plane's products. Requires
+// ny <= EBE_TILE_MAXNY; TILED = false builds the window directly.
+constexpr int EBE_TILE_MAXNY = 128;
+constexpr int EBE_TILE_W = TPB + 2 * EBE_TILE_MAXNY + 2;          // nodes per staged plane
+constexpr int EBE_TILE_NODES_PER_THREAD = (EBE_TILE_W + TPB - 1) / TPB;
+
+template <typename T, typename TS, int SRC>
+__device__ inline void ebe_source(const EbeArgs<T, TS> &a, int jx, int jy, int jz, int64_t node, T (&v)[3]) {
+    if (SRC == SRC_X) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = a.x[3 * node + c];
+    } else if (SRC == SRC_JZ || SRC == SRC_POST) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = a.invD[3 * node + c] * (T)a.r[3 * node + c];
+        if (SRC == SRC_POST) {
+            // trilinear prolongation: 8 terms of weight 1/8; an even fine index
+            // maps both of its terms to the co-located coarse node
+            const int zl = jz >> 1, zh = (jz + 1) >> 1, xl = jx >> 1, xh = (jx + 1) >> 1;
+            const int yl = jy >> 1, yh = (jy + 1) >> 1;
+            const int64_t cpl = (int64_t)a.cnx * a.cny;
+            T pe[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int Z = (t & 4) ? zh : zl, X = (t & 2) ? xh : xl, Y = (t & 1) ? yh : yl;
+                const int64_t J = (int64_t)Z * cpl + (int64_t)X * a.cny + Y;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pe[c] += a.ec[3 * J + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] += a.mask[3 * node + c] ? T(0.125) * pe[c] : T(0);
+        }
+    } else {  // SRC_DIR (T = double)
+        const bool first = a.sc->it == 1;
+        const double beta = a.sc->beta;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            v[c] = first ? (T)a.s[3 * node + c] : (T)(beta * a.dold[3 * node + c] + (double)a.s[3 * node + c]);
+    }
+}
+
+template <typename T, typename TS, int SRC, int EPI, bool TILED>
+__global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
+    if (a.done && *a.done) return;
+    const int nelx = a.nelx, nely = a.nely, nelz = a.nelz;
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t p0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -70,8 +144,8 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int z
     const int64_t p = act ? p0 : plane - 1;
     const int ix = (int)(p / ny);
     const int iy = (int)(p - (int64_t)ix * ny);
-    const int z0 = blockIdx.y * zchunk;
-    const int z1 = min(z0 + zchunk, nz);
+    const int z0 = blockIdx.y * a.zchunk;
+    const int z1 = min(z0 + a.zchunk, nz);
 
     // Neighbour indices are clamped into the grid, so every load is in
     // bounds and needs no predicate; a clamped (nonexistent) neighbour only
@@ -98,38 +172,105 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int z
     T xw[3][3][3][3];  // [z][x][y][c] around (ix, iy, iz)
     T ew[2][2][2];     // [z][x][y] elements around the node
 
-    auto load_plane = [&](int zz, int s) {
-        const int64_t zb = (int64_t)min(max(zz, 0), nz - 1) * plane;
+    // ---- staging of node planes (TILED) ----
+    __shared__ T tile[TILED ? 2 : 1][TILED ? 3 * EBE_TILE_W : 1];
+    const int64_t tile_lo = (int64_t)blockIdx.x * TPB - ny - 1;  // first plane index of the strip
+    const int wn = TPB + 2 * ny + 2;                             // nodes per staged plane
+    T pre[TILED ? EBE_TILE_NODES_PER_THREAD : 1][3];
+    auto fetch_plane = [&](int zz) {                             // sources -> registers
+        const int jz = min(max(zz, 0), nz - 1);
+        const int64_t zb = (int64_t)jz * plane;
+#pragma unroll
+        for (int r = 0; r < EBE_TILE_NODES_PER_THREAD; ++r) {
+            const int w = r * TPB + (int)threadIdx.x;
+            const int64_t pn = min(max(tile_lo + w, (int64_t)0), plane - 1);
+            const int jx = (int)(pn / ny), jy = (int)(pn - (int64_t)jx * ny);
+            if (w < wn) ebe_source<T, TS, SRC>(a, jx, jy, jz, zb + pn, pre[r]);
+        }
+    };
+    auto stash_plane = [&](int buf) {                            // registers -> LDS
+#pragma unroll
+        for (int r = 0; r < EBE_TILE_NODES_PER_THREAD; ++r) {
+            const int w = r * TPB + (int)threadIdx.x;
+            if (w < wn) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) tile[buf][3 * w + c] = pre[r][c];
+            }
+        }
+    };
+    auto window_from_tile = [&](int buf, int s) {                // LDS -> window slot
 #pragma unroll
         for (int xx = 0; xx < 3; ++xx)
 #pragma unroll
             for (int yy = 0; yy < 3; ++yy) {
-                const int64_t nd = zb + (int64_t)cx[xx] * ny + cy[yy];
+                const int t = (int)((int64_t)cx[xx] * ny + cy[yy] - tile_lo);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xw[s][xx][yy][c] = x[3 * nd + c];
+                for (int c = 0; c < 3; ++c) xw[s][xx][yy][c] = tile[buf][3 * t + c];
             }
     };
-    auto load_elems = [&](int ez, int s) {
+    auto load_plane = [&](int zz, int s) {                       // direct (untiled) window load
+        const int jz = min(max(zz, 0), nz - 1);
+        const int64_t zb = (int64_t)jz * plane;
+#pragma unroll
+        for (int xx = 0; xx < 3; ++xx)
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy)
+                ebe_source<T, TS, SRC>(a, cx[xx], cy[yy], jz, zb + (int64_t)cx[xx] * ny + cy[yy], xw[s][xx][yy]);
+    };
+    T epre[2][2];
+    auto fetch_elems = [&](int ez) {
         const T vz = (ez >= 0 && ez < nelz) ? T(1) : T(0);
         const int64_t zb = (int64_t)min(max(ez, 0), nelz - 1) * nelx * nely;
 #pragma unroll
         for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
             for (int oy = 0; oy < 2; ++oy)
-                ew[s][ox][oy] = E[zb + (int64_t)ecx[ox] * nely + ecy[oy]] * (vz * ewxy[ox][oy]);
+                epre[ox][oy] = a.E[zb + (int64_t)ecx[ox] * nely + ecy[oy]] * (vz * ewxy[ox][oy]);
     };
 
-    load_plane(z0 - 1, 0);
-    load_plane(z0, 1);
-    load_elems(z0 - 1, 0);
+    if (TILED) {
+        fetch_plane(z0 - 1);
+        stash_plane(0);
+        fetch_plane(z0);
+        stash_plane(1);
+        __syncthreads();
+        window_from_tile(0, 0);
+        window_from_tile(1, 1);
+        __syncthreads();          // buffer 0 is rewritten in the first step
+        fetch_plane(z0 + 1);
+    } else {
+        load_plane(z0 - 1, 0);
+        load_plane(z0, 1);
+    }
+    fetch_elems(z0 - 1);
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) ew[0][ox][oy] = epre[ox][oy];
+    fetch_elems(z0);
     double dacc = 0.0;
     // Ke is read through the constant address space: the loads are scalar
     // (s_load into SGPRs) and every VALU FMA takes its Ke operand from there.
     typedef const __attribute__((address_space(4))) T *cptr;
-    cptr kp = (cptr)K0;
+    cptr kp = (cptr)a.K0;
     for (int iz = z0; iz < z1; ++iz) {
-        load_plane(iz + 1, 2);
-        load_elems(iz, 1);
+        if (TILED) {
+            // plane iz+1 (fetched during the previous step) into LDS, then into
+            // the window; the LDS buffer alternates so one barrier per step
+            // separates every write from the reads of two steps before
+            const int buf = (iz - z0) & 1;
+            stash_plane(buf);
+            __syncthreads();
+            window_from_tile(buf, 2);
+            fetch_plane(iz + 2);      // in flight during this step's products
+        } else {
+            load_plane(iz + 1, 2);
+        }
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) ew[1][ox][oy] = epre[ox][oy];
+        fetch_elems(iz + 1);
         T y[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int oz = 0; oz < 2; ++oz)
@@ -163,17 +304,30 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int z
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t dof = 3 * node + c;
-                const bool m = mask[dof] != 0;
+                const bool m = a.mask[dof] != 0;
+                const T w = xw[1][1][1][c];
                 if (EPI == EPI_PLAIN) {
-                    out[dof] = m ? y[c] : T(0);
+                    a.out[dof] = m ? y[c] : T(0);
                 } else if (EPI == EPI_RESID) {
-                    out[dof] = m ? (b[dof] - y[c]) : T(0);
+                    a.out[dof] = m ? (a.b[dof] - y[c]) : T(0);
                 } else if (EPI == EPI_JACOBI) {
-                    out[dof] = m ? (xw[1][1][1][c] + invD[dof] * (b[dof] - y[c])) : T(0);
-                } else {
+                    a.out[dof] = m ? (w + a.invD[dof] * (a.b[dof] - y[c])) : T(0);
+                } else if (EPI == EPI_DOT) {
                     const T v = m ? y[c] : T(0);
-                    out[dof] = v;
-                    dacc += (double)xw[1][1][1][c] * (double)v;
+                    a.out[dof] = v;
+                    dacc += (double)w * (double)v;
+                } else if (EPI == EPI_RESIDR) {
+                    a.out[dof] = m ? ((T)a.r[dof] - y[c]) : T(0);
+                } else if (EPI == EPI_POSTR) {
+                    const double rv = a.r[dof];
+                    const T v = m ? (w + a.invD[dof] * ((T)rv - y[c])) : T(0);
+                    a.out[dof] = v;
+                    dacc += rv * (double)v;
+                } else {  // EPI_DIRDOT
+                    const T v = m ? y[c] : T(0);
+                    a.out[dof] = v;
+                    a.dnew[dof] = (double)w;
+                    dacc += (double)w * (double)v;
                 }
             }
         }
@@ -191,7 +345,8 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx, int nely, int nelz, int z
 #pragma unroll
             for (int oy = 0; oy < 2; ++oy) ew[0][ox][oy] = ew[1][ox][oy];
     }
-    if (EPI == EPI_DOT) block_reduce_store(dacc, partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+    if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT)
+        block_reduce_store(dacc, a.partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

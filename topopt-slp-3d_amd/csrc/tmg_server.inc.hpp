@@ -72,7 +72,7 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
     add(8 * ne0); add(4 * ne0); add(8 * ne0);
     add(8 * ne1 * 576); add(8 * (L > 2 ? ne2 * 576 : 1));
     add(8 * ncoarse * ncoarse); add(8 * ncoarse); add(8 * ncoarse); add(4);
-    for (int k = 0; k < 5; ++k) add(8 * ndof0);
+    for (int k = 0; k < 6; ++k) add(8 * ndof0);
     const int64_t pb = (nn0x * nn0y + TPB - 1) / TPB;
     const int64_t nblocks = std::max<int64_t>(pb * nn0z, 4096);  // zchunk >= 1 bound
     add(8 * nblocks); add(8 * nblocks); add(sizeof(PcgScalars));

@@ -60,7 +60,7 @@ struct tmg_solver {
     int *d_info = nullptr;
     rocblas_handle rb = nullptr;
     // PCG
-    double *px = nullptr, *pr = nullptr, *pd = nullptr, *pq = nullptr, *pf = nullptr;
+    double *px = nullptr, *pr = nullptr, *pd = nullptr, *pd2 = nullptr, *pq = nullptr, *pf = nullptr;
     double *partial = nullptr, *partial2 = nullptr;
     int npartial = 0;
     PcgScalars *d_sc = nullptr;
@@ -71,6 +71,8 @@ struct tmg_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;
+    bool ebe_tiled = true;            // LDS-staged fine-level apply (ny <= EBE_TILE_MAXNY)
+    bool fused = true;                // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
     // externally owned resources (a server slot): device memory comes from a
@@ -152,13 +154,35 @@ int64_t apply_blocks(tmg_solver *h, int l) {
     return nblk(lv.nn);
 }
 
+template <typename T, typename TS>
+EbeArgs<T, TS> ebe_args(tmg_solver *h) {
+    const DevLevel &l0 = h->lev[0];
+    EbeArgs<T, TS> a;
+    memset(&a, 0, sizeof(a));
+    a.nelx = l0.nelx; a.nely = l0.nely; a.nelz = l0.nelz; a.zchunk = h->zchunk;
+    a.E = Eof<T>(h); a.K0 = K0of<T>(h); a.mask = l0.mask; a.invD = invof<T>(l0);
+    a.cnx = h->L > 1 ? h->lev[1].nx : 1;
+    a.cny = h->L > 1 ? h->lev[1].ny : 1;
+    return a;
+}
+
+template <typename T, typename TS, int SRC, int EPI>
+void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
+    const DevLevel &lv = h->lev[0];
+    dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + h->zchunk - 1) / h->zchunk);
+    if (lv.ny <= EBE_TILE_MAXNY && h->ebe_tiled)
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, true>), grid, dim3(TPB), 0, h->st, a);
+    else
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false>), grid, dim3(TPB), 0, h->st, a);
+}
+
 template <typename T, int EPI>
 void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial, const int *done) {
     const DevLevel &lv = h->lev[l];
     if (l == 0) {
-        dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + h->zchunk - 1) / h->zchunk);
-        hipLaunchKernelGGL((k_ebe<T, EPI>), grid, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, h->zchunk, x, Eof<T>(h),
-                           K0of<T>(h), lv.mask, b, invof<T>(lv), out, partial, done);
+        EbeArgs<T, T> a = ebe_args<T, T>(h);
+        a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
+        launch_ebe<T, T, SRC_X, EPI>(h, a);
     } else {
         hipLaunchKernelGGL((k_stencil<T, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, x,
                            Sof<T>(lv), lv.mask, b, invof<T>(lv), out, partial, done);
@@ -210,41 +234,91 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done) {
     return xa;
 }
 
+// Fine-level V(1,1) cycle with the first Jacobi sweep, the prolongation of
+// the coarse correction and r's dot product folded into the two fine-level
+// operator applies (Implementation/VCycle.m:23-43 with x0 = 0):
+//   rh = r - A (M\r)                        (sweep from zero + residual)
+//   eH = coarse cycle (P' rh)
+//   s  = w + M\(r - A w), w = M\r + P eH    (prolongation + post-sweep)
+// partial[] receives the per-block sums of r . s. Returns s.
+template <typename T>
+T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
+    DevLevel &l0 = h->lev[0];
+    DevLevel &l1 = h->lev[1];
+    EbeArgs<T, T> a = ebe_args<T, T>(h);
+    a.r = r; a.done = done; a.out = (T *)l0.t;
+    launch_ebe<T, T, SRC_JZ, EPI_RESIDR>(h, a);
+    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(l1.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+                       l1.nz, (const T *)l0.t, l1.mask, (T *)l1.b, done);
+    T *ec;
+    if (h->L == 2) {
+        coarse_solve<T>(h, (const T *)l1.b, (T *)l1.x, done);
+        ec = (T *)l1.x;
+    } else {
+        ec = vcycle<T>(h, 1, (const T *)l1.b, done);
+    }
+    EbeArgs<T, T> b = ebe_args<T, T>(h);
+    b.r = r; b.ec = ec; b.done = done; b.out = (T *)l0.x; b.partial = h->partial;
+    launch_ebe<T, T, SRC_POST, EPI_POSTR>(h, b);
+    return (T *)l0.x;
+}
+
+bool fused_ok(const tmg_solver *h) { return h->fused && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1; }
+
 // Preconditioner application s = B r for the PCG; r is the FP64 residual.
 // Returns s as a typed pointer through one of the two out-params.
 void precondition(tmg_solver *h, const double *r, const double **s64, const float **s32, const int *done) {
     DevLevel &l0 = h->lev[0];
-    if (h->prec == TMG_MIXED) {
+    *s64 = nullptr;
+    *s32 = nullptr;
+    if (fused_ok(h)) {
+        if (h->prec == TMG_MIXED) *s32 = vcycle_fine_fused<float>(h, r, done);
+        else *s64 = vcycle_fine_fused<double>(h, r, done);
+    } else if (h->prec == TMG_MIXED) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, (float *)l0.b, done);
         *s32 = vcycle<float>(h, 0, (const float *)l0.b, done);
-        *s64 = nullptr;
     } else {
         *s64 = vcycle<double>(h, 0, r, done);
-        *s32 = nullptr;
     }
 }
 
+// q = A d with d = beta d_old + s formed on the fly; writes d_new, partial d.q.
+template <typename TS>
+void direction_apply(tmg_solver *h, const TS *s, const double *dold, double *dnew, const int *done) {
+    EbeArgs<double, TS> a = ebe_args<double, TS>(h);
+    a.s = s; a.dold = dold; a.dnew = dnew; a.sc = h->d_sc; a.out = h->pq; a.partial = h->partial; a.done = done;
+    launch_ebe<double, TS, SRC_DIR, EPI_DIRDOT>(h, a);
+}
+
 // One PCG iteration of Implementation/SolveMG.m:30-52, all scalars on device.
-void pcg_iteration(tmg_solver *h) {
+// parity selects which of the two direction buffers holds d_old.
+void pcg_iteration(tmg_solver *h, int parity) {
     const int *done = &h->d_sc->done;
     const int64_t n = h->lev[0].ndof;
     const double *s64;
     const float *s32;
-    precondition(h, h->pr, &s64, &s32, done);
     const int nb = h->npartial;
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), nb);
-    if (s32) {
-        hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
+    const int napply = (int)apply_blocks(h, 0);
+    precondition(h, h->pr, &s64, &s32, done);
+    if (fused_ok(h)) {
+        // rho = r' s (partials from the post-smoothing apply)
+        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
+        double *dold = parity ? h->pd2 : h->pd, *dnew = parity ? h->pd : h->pd2;
+        if (s32) direction_apply<float>(h, s32, dold, dnew, done);
+        else direction_apply<double>(h, s64, dold, dnew, done);
+        hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
+        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, dnew, h->pq, h->partial, h->d_sc);
     } else {
-        hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
+        if (s32) hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
+        else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
+        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+        if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
+        else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
+        apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);          // q = A d, d'q
+        hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
+        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial, h->d_sc);
     }
-    hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
-    if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
-    else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
-    apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);          // q = A d, d'q
-    hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, (int)apply_blocks(h, 0), h->partial, 0,
-                       nullptr, h->d_sc);
-    hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial, h->d_sc);
     hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
 }
 
@@ -272,6 +346,7 @@ int solve_begin(tmg_solver *h) {
     HIPX(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, h->st));
     // d = 0 so that the first direction update is exact
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd);
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd2);
     // r = f - A x0 (Implementation/SolveMG.m:26-27)
     apply<double, EPI_RESID>(h, 0, h->px, h->pf, h->pr, nullptr, nullptr);
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), h->npartial);
@@ -284,7 +359,7 @@ int solve_begin(tmg_solver *h) {
         // capture GRAPH_ITERS consecutive iterations; iterations past
         // convergence are skipped on the device through the done flag
         HIPX(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < GRAPH_ITERS; ++k) pcg_iteration(h);
+        for (int k = 0; k < GRAPH_ITERS; ++k) pcg_iteration(h, k & 1);
         hipGraph_t g = nullptr;
         HIPX(hipStreamEndCapture(h->st, &g));
         h->graph = g;
@@ -300,7 +375,7 @@ int solve_launch(tmg_solver *h, int nchunk) {
     if (h->graph_exec) {
         for (int k = 0; k < nchunk; ++k) HIPX(hipGraphLaunch(h->graph_exec, h->st));
     } else {
-        for (int k = 0; k < nchunk * GRAPH_ITERS; ++k) pcg_iteration(h);
+        for (int k = 0; k < nchunk * GRAPH_ITERS; ++k) pcg_iteration(h, k & 1);
     }
     TRY(check_launch(h));
     HIPX(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->st));
@@ -387,7 +462,7 @@ void destroy_all(tmg_solver *h) {
     dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
     dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
-    dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pq); dfree(h, h->pf);
+    dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
     dfree(h, h->partial); dfree(h, h->partial2); dfree(h, h->d_sc);
     if (h->h_sc && h->own_hsc) (void)hipHostFree(h->h_sc);
     h->h_sc = nullptr;
@@ -575,6 +650,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->prec = precision;
     h->L = mg->ngrids;
     if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
+    if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
+    if (const char *ev = getenv("TMG_UNFUSED")) h->fused = ev[0] == '0';
     if (ext) {
         h->st = ext->st;
         h->rb = ext->rb;
@@ -642,6 +719,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
         (rc = dalloc(h, &h->px, l0.ndof)) || (rc = dalloc(h, &h->pr, l0.ndof)) || (rc = dalloc(h, &h->pd, l0.ndof)) ||
+        (rc = dalloc(h, &h->pd2, l0.ndof)) ||
         (rc = dalloc(h, &h->pq, l0.ndof)) || (rc = dalloc(h, &h->pf, l0.ndof)))
         return fail(rc);
     // z-chunk of the fine apply: enough blocks to fill the chip
@@ -839,14 +917,19 @@ int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t 
 int tmg_ngrids(const tmg_solver *h) { return h ? h->L : 0; }
 
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
-    if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 3) return TMG_ERR_ARG;
+    if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 6) return TMG_ERR_ARG;
     if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
     if (which == 2 && h->L < 3) return set_err(h, TMG_ERR_ARG, "level 2 is the coarsest level here");
     DevLevel &l0 = h->lev[0];
     const bool mixed = h->prec == TMG_MIXED;
-    // deterministic non-trivial inputs: x = f-like ones pattern on free DOFs
+    // deterministic non-trivial inputs
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pd);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pr);
+    HIPX(hipStreamSynchronize(h->st));
+    memset(h->h_sc, 0, sizeof(PcgScalars));
+    h->h_sc->it = 2;
+    h->h_sc->beta = 0.5;
+    HIPX(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, h->st));
     auto launch = [&]() {
         if (which == 0) {
             apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, nullptr);
@@ -855,10 +938,25 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
             DevLevel &lv = h->lev[l];
             if (mixed) apply<float, EPI_JACOBI>(h, l, (const float *)lv.x, (const float *)lv.b, (float *)lv.t, nullptr, nullptr);
             else apply<double, EPI_JACOBI>(h, l, (const double *)lv.x, (const double *)lv.b, (double *)lv.t, nullptr, nullptr);
-        } else {
+        } else if (which == 3) {
             const double *s64;
             const float *s32;
             precondition(h, h->pr, &s64, &s32, nullptr);
+        } else if (which == 4 || which == 5) {
+            if (mixed) {
+                EbeArgs<float, float> a = ebe_args<float, float>(h);
+                a.r = h->pr; a.ec = (const float *)h->lev[1].x; a.out = (float *)l0.t; a.partial = h->partial;
+                if (which == 4) launch_ebe<float, float, SRC_JZ, EPI_RESIDR>(h, a);
+                else launch_ebe<float, float, SRC_POST, EPI_POSTR>(h, a);
+            } else {
+                EbeArgs<double, double> a = ebe_args<double, double>(h);
+                a.r = h->pr; a.ec = (const double *)h->lev[1].x; a.out = (double *)l0.t; a.partial = h->partial;
+                if (which == 4) launch_ebe<double, double, SRC_JZ, EPI_RESIDR>(h, a);
+                else launch_ebe<double, double, SRC_POST, EPI_POSTR>(h, a);
+            }
+        } else {
+            if (mixed) direction_apply<float>(h, (const float *)l0.x, h->pd, h->pd2, nullptr);
+            else direction_apply<double>(h, (const double *)l0.x, h->pd, h->pd2, nullptr);
         }
     };
     // level vectors need finite values for the sweeps
