@@ -83,6 +83,15 @@ typedef struct tmg_mg_params {
  * the level rules of Implementation/StartStrSLP.m:107-133. */
 int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, tmg_solver **out);
 
+/* Multiresolution variant of tmg_setup (mr.op = true, mr.n = mr.d = 2,
+ * mr.interp = false): each displacement element carries 2x2x2 density
+ * sub-elements and K_e = sum_s (emin + (E0-emin) x_s^p) K_s with the
+ * sub-element integrands of Implementation/ElemStiffMatrixGQMR.m
+ * (Implementation/GlobalStiffMatrix.m:19-62, Implementation/WOK/KelemProd.m:58-83).
+ * Densities (tmg_update_density) and gradients (tmg_grad) then live on the
+ * density mesh (2nelx x 2nely x 2nelz, y fastest). mr_n = 1 is tmg_setup. */
+int tmg_setup_mr(const tmg_problem *prob, const tmg_mg_params *mg, int precision, int mr_n, tmg_solver **out);
+
 /* Operator refresh for new filtered densities x (n_el, element order):
  * K = sum_e (emin + (E0-emin) x_e^p) Ke, Galerkin A{i} = P'A{i-1}P, Jacobi
  * M{i} = (1/omega) diag(A{i}), coarse Cholesky. Replaces
@@ -129,8 +138,9 @@ int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc);
 int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t *freedofs1);
 int tmg_ngrids(const tmg_solver *h);
 
-/* Fgrad = Grad(str,u,p,emin,k,x,Dofs,W,w,opfilter,mr,strDens), single
- * resolution (Implementation/Grad.m:18-23,62). u: full-length displacement
+/* Fgrad = Grad(str,u,p,emin,k,x,Dofs,W,w,opfilter,mr,strDens)
+ * (Implementation/Grad.m:18-62; with a multiresolution handle x, Fgrad and
+ * the filter are on the density mesh, mr.interp = false). u: full-length displacement
  * (3*nnodes, fixed DOFs 0); x: filtered densities (n_el); freedens: optional
  * 0/1 element mask (str.freeDensG; NULL = all); filt: the filter (NULL =
  * opfilter 0). Fgrad: n_el. All host pointers. */
@@ -148,6 +158,17 @@ int tmg_filter_apply(tmg_filter *f, const double *x, int trans, double *y);
 /* w = full(sum(W,2)), the row sums of W (Implementation/ElemNeighborhood.m:98). */
 int tmg_filter_rowsum(const tmg_filter *f, double *w);
 void tmg_filter_destroy(tmg_filter *f);
+
+/* The LP of one SLP step, [s,~,~,~,lambda] = linprog(c,A,-b,[],[],sL,sU)
+ * (Implementation/StrSLP.m:265-269; equality form with Aeq at :271): minimise
+ * c's subject to the single row a's <= r (equality != 0: a's = r) and
+ * sL <= s <= sU. Solved exactly as a continuous knapsack on the host.
+ * lambda receives the row's multiplier (lambda.ineqlin / lambda.eqlin).
+ * Returns TMG_OK, or TMG_NOT_CONVERGED when the row cannot be met inside the
+ * box (s is then the box point closest to feasibility), or TMG_ERR_ARG.
+ * Host vectors of n. */
+int tmg_lp_volume(int64_t n, const double *c, const double *a, const double *sL, const double *sU, double r,
+                  int equality, double *s, double *lambda);
 
 /* Instrumentation (no counterpart in the reference): average device time in
  * milliseconds of one launch of a solver kernel, measured with HIP events on

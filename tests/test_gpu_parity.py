@@ -214,3 +214,36 @@ def test_error_paths():
     s.update_density(np.full(prob.nelem, 0.5))
     with pytest.raises(tmg.TmgError):
         s.apply(9, np.zeros(3))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multiresolution_matches_oracle(oracle_mod, precision):
+    """mr.n = 2: operator, Galerkin hierarchy, solve and density-mesh gradient
+    against the oracle's multiresolution assembly (Implementation/GlobalStiffMatrix.m:19-62,
+    Implementation/Grad.m:24-49)."""
+    _say(f"multiresolution {precision}")
+    import tmg
+    nelx, nely, nelz, L = 8, 4, 4, 3
+    prob = P.cantilever(nelx, nely, nelz)
+    xd = P.random_density(2 * nelx, 2 * nely, 2 * nelz, 21)
+    o = oracle_mod.Oracle(prob)
+    o.assemble_mr(2, xd)
+    o.setup_gmg(L, 0.6)
+    s = tmg.Solver(prob, L, precision=precision, mr=2)
+    s.update_density(xd)
+    rng = np.random.default_rng(3)
+    for lev in range(1, L + 1):
+        v = rng.standard_normal(o.nfree(lev - 1))
+        assert _rel(s.apply(lev, v), o.spmv(lev - 1, v)) < 1e-13
+    u_or, it_or, _ = o.solve(prob.f)
+    u, it, rr, rc = s.solve(prob.f)
+    assert rc == 0
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    ufull = prob.full(u_or)
+    assert _rel(s.grad(ufull, xd), o.grad0_mr(2, ufull, xd)) < 1e-12
+    f = tmg.Filter(2 * nelx, 2 * nely, 2 * nelz, 1.5, 2, el=0.5, eh=0.5, ew=0.5)
+    W, w = oracle_mod.filter_matrix(2 * nelx, 2 * nely, 2 * nelz, 1.5, 2, el=0.5, eh=0.5, ew=0.5)
+    g0 = o.grad0_mr(2, ufull, xd)
+    assert _rel(s.grad(ufull, xd, filt=f), W.T @ (g0 / w)) < 1e-12

@@ -95,7 +95,7 @@ struct EbeArgs {
 // (source transform included) and loaded cooperatively, and the next plane's
 // loads are issued before the current plane's products. Requires
 // ny <= EBE_TILE_MAXNY; TILED = false builds the window directly.
-constexpr int EBE_TILE_MAXNY = 128;
+constexpr int EBE_TILE_MAXNY = 127;
 constexpr int EBE_TILE_W = TPB + 2 * EBE_TILE_MAXNY + 2;          // nodes per staged plane
 constexpr int EBE_TILE_NODES_PER_THREAD = (EBE_TILE_W + TPB - 1) / TPB;
 
@@ -133,9 +133,25 @@ __device__ inline void ebe_source(const EbeArgs<T, TS> &a, int jx, int jy, int j
     }
 }
 
-template <typename T, typename TS, int SRC, int EPI, bool TILED>
-__global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
-    if (a.done && *a.done) return;
+// NSUB = 1: one modulus per element (E[e]); NSUB = 8: multiresolution with
+// 2x2x2 density sub-elements, E[8e + s] and eight sub-element integrands
+// K0[576 s + ...] (s = sy + 2 sx + 4 sz), element matrix sum_s E_s K_s.
+template <typename T, typename TS, int SRC, int EPI, bool TILED, int NSUB>
+__global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, int zchunk_, const T *__restrict__ x_,
+                                             const T *__restrict__ E_, const T *__restrict__ K0_,
+                                             const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
+                                             const T *__restrict__ invD_, const double *__restrict__ r_,
+                                             const T *__restrict__ ec_, int cnx_, int cny_, const TS *__restrict__ s_,
+                                             const double *__restrict__ dold_, double *__restrict__ dnew_,
+                                             const PcgScalars *__restrict__ sc_, T *__restrict__ out_,
+                                             double *__restrict__ partial_, const int *__restrict__ done_) {
+    if (done_ && *done_) return;
+    // the arguments arrive as non-aliasing kernel parameters and are bundled
+    // here only for the helpers
+    EbeArgs<T, TS> a;
+    a.nelx = nelx_; a.nely = nely_; a.nelz = nelz_; a.zchunk = zchunk_; a.x = x_; a.E = E_; a.K0 = K0_;
+    a.mask = mask_; a.b = b_; a.invD = invD_; a.r = r_; a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.s = s_;
+    a.dold = dold_; a.dnew = dnew_; a.sc = sc_; a.out = out_; a.partial = partial_; a.done = done_;
     const int nelx = a.nelx, nely = a.nely, nelz = a.nelz;
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t plane = (int64_t)nx * ny;
@@ -169,8 +185,11 @@ __global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
         for (int oy = 0; oy < 2; ++oy)
             ewxy[ox][oy] = (ix - 1 + ox >= 0 && ix - 1 + ox < nelx && iy - 1 + oy >= 0 && iy - 1 + oy < nely) ? T(1) : T(0);
 
-    T xw[3][3][3][3];  // [z][x][y][c] around (ix, iy, iz)
-    T ew[2][2][2];     // [z][x][y] elements around the node
+    // Element layers are streamed in z: layer L (between node planes L and
+    // L+1) adds to its bottom node (plane L), which is then complete, and
+    // starts its top node (plane L+1). Only two node planes are live.
+    T xw[2][3][3][3];  // [bottom/top plane][x][y][c] around the column
+    T ew[2][2][NSUB];  // moduli of the four elements of the current layer
 
     // ---- staging of node planes (TILED) ----
     __shared__ T tile[TILED ? 2 : 1][TILED ? 3 * EBE_TILE_W : 1];
@@ -198,133 +217,159 @@ __global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
             }
         }
     };
-    auto window_from_tile = [&](int buf, int s) {                // LDS -> window slot
+    auto window_from_tile = [&](int buf) {                       // LDS -> top plane of the window
 #pragma unroll
         for (int xx = 0; xx < 3; ++xx)
 #pragma unroll
             for (int yy = 0; yy < 3; ++yy) {
                 const int t = (int)((int64_t)cx[xx] * ny + cy[yy] - tile_lo);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xw[s][xx][yy][c] = tile[buf][3 * t + c];
+                for (int c = 0; c < 3; ++c) xw[1][xx][yy][c] = tile[buf][3 * t + c];
             }
     };
-    auto load_plane = [&](int zz, int s) {                       // direct (untiled) window load
+    auto load_plane = [&](int zz) {                              // direct (untiled) top-plane load
         const int jz = min(max(zz, 0), nz - 1);
         const int64_t zb = (int64_t)jz * plane;
 #pragma unroll
         for (int xx = 0; xx < 3; ++xx)
 #pragma unroll
             for (int yy = 0; yy < 3; ++yy)
-                ebe_source<T, TS, SRC>(a, cx[xx], cy[yy], jz, zb + (int64_t)cx[xx] * ny + cy[yy], xw[s][xx][yy]);
+                ebe_source<T, TS, SRC>(a, cx[xx], cy[yy], jz, zb + (int64_t)cx[xx] * ny + cy[yy], xw[1][xx][yy]);
     };
-    T epre[2][2];
-    auto fetch_elems = [&](int ez) {
+    // moduli of layer ez; single resolution prefetches one layer ahead,
+    // multiresolution (8x the values) loads at use
+    T epre[2][2][1];
+    auto fetch_elems = [&](int ez, T (&dst)[2][2][1]) {
         const T vz = (ez >= 0 && ez < nelz) ? T(1) : T(0);
         const int64_t zb = (int64_t)min(max(ez, 0), nelz - 1) * nelx * nely;
 #pragma unroll
         for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
             for (int oy = 0; oy < 2; ++oy)
-                epre[ox][oy] = a.E[zb + (int64_t)ecx[ox] * nely + ecy[oy]] * (vz * ewxy[ox][oy]);
+                dst[ox][oy][0] = a.E[zb + (int64_t)ecx[ox] * nely + ecy[oy]] * (vz * ewxy[ox][oy]);
+    };
+    auto load_elems_mr = [&](int ez) {
+        const T vz = (ez >= 0 && ez < nelz) ? T(1) : T(0);
+        const int64_t zb = (int64_t)min(max(ez, 0), nelz - 1) * nelx * nely;
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) {
+                const int64_t e = zb + (int64_t)ecx[ox] * nely + ecy[oy];
+#pragma unroll
+                for (int s = 0; s < NSUB; ++s) ew[ox][oy][s] = a.E[NSUB * e + s] * (vz * ewxy[ox][oy]);
+            }
     };
 
+    // plane z0 - 1 into the top slot (it becomes the bottom of layer z0 - 1)
     if (TILED) {
         fetch_plane(z0 - 1);
         stash_plane(0);
-        fetch_plane(z0);
-        stash_plane(1);
         __syncthreads();
-        window_from_tile(0, 0);
-        window_from_tile(1, 1);
-        __syncthreads();          // buffer 0 is rewritten in the first step
-        fetch_plane(z0 + 1);
+        window_from_tile(0);
+        fetch_plane(z0);
     } else {
-        load_plane(z0 - 1, 0);
-        load_plane(z0, 1);
+        load_plane(z0 - 1);
     }
-    fetch_elems(z0 - 1);
-#pragma unroll
-    for (int ox = 0; ox < 2; ++ox)
-#pragma unroll
-        for (int oy = 0; oy < 2; ++oy) ew[0][ox][oy] = epre[ox][oy];
-    fetch_elems(z0);
+    if (NSUB == 1) fetch_elems(z0 - 1, epre);
     double dacc = 0.0;
+    T ybot[3] = {T(0), T(0), T(0)};   // node L   (completed by layer L)
     // Ke is read through the constant address space: the loads are scalar
     // (s_load into SGPRs) and every VALU FMA takes its Ke operand from there.
     typedef const __attribute__((address_space(4))) T *cptr;
     cptr kp = (cptr)a.K0;
-    for (int iz = z0; iz < z1; ++iz) {
+    for (int L = z0 - 1; L < z1; ++L) {
+        // shift: the top plane becomes the bottom one; plane L+1 is the new top
+#pragma unroll
+        for (int xx = 0; xx < 3; ++xx)
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xw[0][xx][yy][c] = xw[1][xx][yy][c];
         if (TILED) {
-            // plane iz+1 (fetched during the previous step) into LDS, then into
-            // the window; the LDS buffer alternates so one barrier per step
-            // separates every write from the reads of two steps before
-            const int buf = (iz - z0) & 1;
+            // plane L+1 (fetched during the previous step) into LDS, then into
+            // the window; buffers alternate so one barrier per step separates
+            // every write from the reads of two steps before
+            const int buf = (L - z0 + 2) & 1;
             stash_plane(buf);
             __syncthreads();
-            window_from_tile(buf, 2);
-            fetch_plane(iz + 2);      // in flight during this step's products
+            window_from_tile(buf);
+            fetch_plane(L + 2);       // in flight during this step's products
         } else {
-            load_plane(iz + 1, 2);
+            load_plane(L + 1);
         }
-#pragma unroll
-        for (int ox = 0; ox < 2; ++ox)
-#pragma unroll
-            for (int oy = 0; oy < 2; ++oy) ew[1][ox][oy] = epre[ox][oy];
-        fetch_elems(iz + 1);
-        T y[3] = {T(0), T(0), T(0)};
-#pragma unroll
-        for (int oz = 0; oz < 2; ++oz)
+        if (NSUB == 1) {
 #pragma unroll
             for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
-                for (int oy = 0; oy < 2; ++oy) {
-                    // this node is local node ai of the element at offset (ox,oy,oz)
-                    const int ai = (1 - oy) + 2 * (1 - ox) + 4 * (1 - oz);
-                    // rows 3*ai+c of Ke are consumed contiguously (24 scalars each)
-                    const T Ee = ew[oz][ox][oy];
+                for (int oy = 0; oy < 2; ++oy) ew[ox][oy][0] = epre[ox][oy][0];
+            fetch_elems(L + 1, epre);
+        } else {
+            load_elems_mr(L);
+        }
+        const bool do_bot = L >= z0;        // node L belongs to this chunk
+        const bool do_top = L + 1 < z1;     // node L+1 belongs to this chunk
+        T ytop[3] = {T(0), T(0), T(0)};
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        // Re-materialise the Ke base per row, ordered after the
-                        // previous row's result, so one row (24 scalars) is live
-                        // at a time: otherwise the loop-invariant scalar loads
-                        // are hoisted en bloc and spill to VGPR lanes.
-                        asm volatile("" : "+s"(kp) : "v"(y[(c + 2) % 3]));
-                        T t = T(0);
+        for (int ox = 0; ox < 2; ++ox)
 #pragma unroll
-                        for (int j = 0; j < 24; ++j) {
-                            const int bn = j / 3, d = j % 3;
-                            const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
-                            t += kp[(3 * ai + c) * 24 + j] * xw[oz + bz][ox + bx][oy + by][d];
+            for (int oy = 0; oy < 2; ++oy) {
+                // the element at offset (ox, oy) of layer L: the column node is
+                // its local node (1-oy) + 2(1-ox) at the bottom, +4 at the top
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    if (half == 0 ? !do_bot : !do_top) continue;
+                    const int ai = (1 - oy) + 2 * (1 - ox) + 4 * half;
+#pragma unroll
+                    for (int s = 0; s < NSUB; ++s) {
+                        const T Ee = ew[ox][oy][s];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            // Re-materialise the Ke base per row, ordered after
+                            // the previous row's result, so one row (24 scalars)
+                            // is live at a time: otherwise the loop-invariant
+                            // scalar loads are hoisted en bloc and spill.
+                            asm volatile("" : "+s"(kp) : "v"(half ? ytop[(c + 2) % 3] : ybot[(c + 2) % 3]));
+                            T t = T(0);
+#pragma unroll
+                            for (int j = 0; j < 24; ++j) {
+                                const int bn = j / 3, d = j % 3;
+                                const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
+                                t += kp[576 * s + (3 * ai + c) * 24 + j] * xw[bz][ox + bx][oy + by][d];
+                            }
+                            if (half == 0) ybot[c] += Ee * t;
+                            else ytop[c] += Ee * t;
                         }
-                        y[c] += Ee * t;
                     }
                 }
-        if (act) {
-            const int64_t node = (int64_t)iz * plane + p;
+            }
+        if (do_bot && act) {
+            const int64_t node = (int64_t)L * plane + p;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t dof = 3 * node + c;
                 const bool m = a.mask[dof] != 0;
-                const T w = xw[1][1][1][c];
+                const T w = xw[0][1][1][c];
+                const T y = ybot[c];
                 if (EPI == EPI_PLAIN) {
-                    a.out[dof] = m ? y[c] : T(0);
+                    a.out[dof] = m ? y : T(0);
                 } else if (EPI == EPI_RESID) {
-                    a.out[dof] = m ? (a.b[dof] - y[c]) : T(0);
+                    a.out[dof] = m ? (a.b[dof] - y) : T(0);
                 } else if (EPI == EPI_JACOBI) {
-                    a.out[dof] = m ? (w + a.invD[dof] * (a.b[dof] - y[c])) : T(0);
+                    a.out[dof] = m ? (w + a.invD[dof] * (a.b[dof] - y)) : T(0);
                 } else if (EPI == EPI_DOT) {
-                    const T v = m ? y[c] : T(0);
+                    const T v = m ? y : T(0);
                     a.out[dof] = v;
                     dacc += (double)w * (double)v;
                 } else if (EPI == EPI_RESIDR) {
-                    a.out[dof] = m ? ((T)a.r[dof] - y[c]) : T(0);
+                    a.out[dof] = m ? ((T)a.r[dof] - y) : T(0);
                 } else if (EPI == EPI_POSTR) {
                     const double rv = a.r[dof];
-                    const T v = m ? (w + a.invD[dof] * ((T)rv - y[c])) : T(0);
+                    const T v = m ? (w + a.invD[dof] * ((T)rv - y)) : T(0);
                     a.out[dof] = v;
                     dacc += rv * (double)v;
                 } else {  // EPI_DIRDOT
-                    const T v = m ? y[c] : T(0);
+                    const T v = m ? y : T(0);
                     a.out[dof] = v;
                     a.dnew[dof] = (double)w;
                     dacc += (double)w * (double)v;
@@ -332,18 +377,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
             }
         }
 #pragma unroll
-        for (int xx = 0; xx < 3; ++xx)
-#pragma unroll
-            for (int yy = 0; yy < 3; ++yy)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    xw[0][xx][yy][c] = xw[1][xx][yy][c];
-                    xw[1][xx][yy][c] = xw[2][xx][yy][c];
-                }
-#pragma unroll
-        for (int ox = 0; ox < 2; ++ox)
-#pragma unroll
-            for (int oy = 0; oy < 2; ++oy) ew[0][ox][oy] = ew[1][ox][oy];
+        for (int c = 0; c < 3; ++c) ybot[c] = ytop[c];
     }
     if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT)
         block_reduce_store(dacc, a.partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
@@ -363,32 +397,79 @@ __global__ __launch_bounds__(TPB) void k_ebe(EbeArgs<T, TS> a) {
 constexpr int NCOEF = 126;
 __host__ __device__ inline int64_t sidx(int64_t i, int coef) { return (i >> 6) * (NCOEF * 64) + coef * 64 + (i & 63); }
 
+template <typename T>
+struct StArgs {
+    int nx, ny, nz;
+    const T *x;            // SRC_X input
+    const T *S;            // symmetric block stencil
+    const uint8_t *mask;
+    const T *b;            // right-hand side
+    const T *invD;
+    const T *ec;           // next-coarser level's correction (SRC_POST)
+    int cnx, cny;
+    T *out;
+    double *partial;
+    const int *done;
+};
+
+// Source value of node j (SRC_X: x; SRC_JZ: M\b, the first Jacobi sweep
+// from zero; SRC_POST: M\b + P e_c, that sweep plus the prolonged coarse
+// correction).
+template <typename T, int SRC>
+__device__ inline void stencil_source(const StArgs<T> &a, int jx, int jy, int jz, int64_t j, T (&v)[3]) {
+    if (SRC == SRC_X) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = a.x[3 * j + d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = a.invD[3 * j + d] * a.b[3 * j + d];
+        if (SRC == SRC_POST) {
+            const int zl = jz >> 1, zh = (jz + 1) >> 1, xl = jx >> 1, xh = (jx + 1) >> 1;
+            const int yl = jy >> 1, yh = (jy + 1) >> 1;
+            const int64_t cpl = (int64_t)a.cnx * a.cny;
+            T pe[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int Z = (t & 4) ? zh : zl, X = (t & 2) ? xh : xl, Y = (t & 1) ? yh : yl;
+                const int64_t J = (int64_t)Z * cpl + (int64_t)X * a.cny + Y;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) pe[d] += a.ec[3 * J + d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] += a.mask[3 * j + d] ? T(0.125) * pe[d] : T(0);
+        }
+    }
+}
+
 // y += contributions of the 9 neighbours in plane dz = DZ.
-template <typename T, int DZ>
-__device__ inline void stencil_plane(int64_t i, int ix, int iy, int iz, int nx, int ny, int nz, int64_t plane,
-                                     const T *__restrict__ x, const T *__restrict__ S, T (&y)[3]) {
+template <typename T, int SRC, int DZ>
+__device__ inline void stencil_plane(const StArgs<T> &a, int64_t i, int ix, int iy, int iz, int64_t plane, T (&y)[3]) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int jz = iz + DZ;
     const bool vz = jz >= 0 && jz < nz;
-    const int64_t jzb = (int64_t)min(max(jz, 0), nz - 1) * plane;
+    const int jzc = min(max(jz, 0), nz - 1);
+    const int64_t jzb = (int64_t)jzc * plane;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const int dx = q / 3 - 1, dy = q % 3 - 1;
         const int k = (DZ + 1) * 9 + q;
         const int jx = ix + dx, jy = iy + dy;
         const bool v = vz && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
-        const int64_t j = jzb + (int64_t)min(max(jx, 0), nx - 1) * ny + min(max(jy, 0), ny - 1);
+        const int jxc = min(max(jx, 0), nx - 1), jyc = min(max(jy, 0), ny - 1);
+        const int64_t j = jzb + (int64_t)jxc * ny + jyc;
         const T vf = v ? T(1) : T(0);   // a clamped neighbour contributes nothing
         T xv[3];
+        stencil_source<T, SRC>(a, jxc, jyc, jzc, j, xv);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) xv[d] = x[3 * j + d] * vf;
+        for (int d = 0; d < 3; ++d) xv[d] *= vf;
         if (k >= 13) {
-            const T *Sw = S + sidx(i, (k - 13) * 9);
+            const T *Sw = a.S + sidx(i, (k - 13) * 9);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) y[c] += Sw[(c * 3 + d) * 64] * xv[d];
         } else {
-            const T *Sw = S + sidx(j, (13 - k) * 9);   // (A_ji)^T
+            const T *Sw = a.S + sidx(j, (13 - k) * 9);   // (A_ji)^T
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -397,13 +478,19 @@ __device__ inline void stencil_plane(int64_t i, int ix, int iy, int iz, int nx, 
     }
 }
 
-template <typename T, int EPI>
-__global__ __launch_bounds__(TPB) void k_stencil(int nx, int ny, int nz, const T *__restrict__ x,
-                                                 const T *__restrict__ S, const uint8_t *__restrict__ mask,
-                                                 const T *__restrict__ b, const T *__restrict__ invD,
-                                                 T *__restrict__ out, double *__restrict__ partial,
-                                                 const int *__restrict__ done) {
-    if (done && *done) return;
+template <typename T, int SRC, int EPI>
+__global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, const T *__restrict__ x_, const T *__restrict__ S_,
+                                                 const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
+                                                 const T *__restrict__ invD_, const T *__restrict__ ec_, int cnx_, int cny_,
+                                                 T *__restrict__ out_, double *__restrict__ partial_,
+                                                 const int *__restrict__ done_) {
+    if (done_ && *done_) return;
+    // the arguments arrive as non-aliasing kernel parameters and are bundled
+    // here only for the helpers
+    StArgs<T> a;
+    a.nx = nx_; a.ny = ny_; a.nz = nz_; a.x = x_; a.S = S_; a.mask = mask_; a.b = b_; a.invD = invD_;
+    a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.out = out_; a.partial = partial_; a.done = done_;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t nn = plane * nz;
     const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -418,29 +505,31 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx, int ny, int nz, const T
         // one plane of neighbours at a time: opaque copies of the coordinates
         // keep the next plane's loads from being hoisted (and held in
         // registers) before this plane's products are done
-        stencil_plane<T, -1>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
+        stencil_plane<T, SRC, -1>(a, i, ix, iy, iz, plane, y);
         asm volatile("" : "+v"(i), "+v"(ix), "+v"(iy), "+v"(iz) : "v"(y[0]), "v"(y[1]), "v"(y[2]));
-        stencil_plane<T, 0>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
+        stencil_plane<T, SRC, 0>(a, i, ix, iy, iz, plane, y);
         asm volatile("" : "+v"(i), "+v"(ix), "+v"(iy), "+v"(iz) : "v"(y[0]), "v"(y[1]), "v"(y[2]));
-        stencil_plane<T, 1>(i, ix, iy, iz, nx, ny, nz, plane, x, S, y);
+        stencil_plane<T, SRC, 1>(a, i, ix, iy, iz, plane, y);
+        T w[3];
+        stencil_source<T, SRC>(a, (int)((i0 / ny) % nx), (int)(i0 % ny), (int)(i0 / plane), i0, w);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int64_t dof = 3 * i0 + c;
-            const bool m = mask[dof] != 0;
+            const bool m = a.mask[dof] != 0;
             if (EPI == EPI_PLAIN) {
-                out[dof] = m ? y[c] : T(0);
+                a.out[dof] = m ? y[c] : T(0);
             } else if (EPI == EPI_RESID) {
-                out[dof] = m ? (b[dof] - y[c]) : T(0);
+                a.out[dof] = m ? (a.b[dof] - y[c]) : T(0);
             } else if (EPI == EPI_JACOBI) {
-                out[dof] = m ? (x[dof] + invD[dof] * (b[dof] - y[c])) : T(0);
+                a.out[dof] = m ? (w[c] + a.invD[dof] * (a.b[dof] - y[c])) : T(0);
             } else {
                 const T v = m ? y[c] : T(0);
-                out[dof] = v;
-                dacc += (double)x[dof] * (double)v;
+                a.out[dof] = v;
+                dacc += (double)w[c] * (double)v;
             }
         }
     }
-    if (EPI == EPI_DOT) block_reduce_store(dacc, partial, blockIdx.x);
+    if (EPI == EPI_DOT) block_reduce_store(dacc, a.partial, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -634,6 +723,33 @@ __global__ __launch_bounds__(TPB) void k_scatter_full(int64_t ndof, const int64_
     out[d] = q >= 0 ? (T)v[q] : T(0);
 }
 
+// Coarsest-level solve in one launch: x[free[q]] = sum_j Ainv[q][j] b[free[j]],
+// one wave per row, four independent partial sums per lane. Fixed DOFs of x
+// are never written (they stay zero).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const double *__restrict__ A,
+                                                      const int64_t *__restrict__ freeidx, const T *__restrict__ b,
+                                                      T *__restrict__ x, const int *done) {
+    if (done && *done) return;
+    const int64_t row = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double *Ar = A + row * n;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t j = lane;
+    for (; j + 192 < n; j += 256) {
+        s0 += Ar[j] * (double)b[freeidx[j]];
+        s1 += Ar[j + 64] * (double)b[freeidx[j + 64]];
+        s2 += Ar[j + 128] * (double)b[freeidx[j + 128]];
+        s3 += Ar[j + 192] * (double)b[freeidx[j + 192]];
+    }
+    for (; j < n; j += 64) s0 += Ar[j] * (double)b[freeidx[j]];
+    double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[freeidx[row]] = (T)s;
+}
+
 __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,
                                               double *__restrict__ y, const int *done) {
     if (done && *done) return;
@@ -660,9 +776,29 @@ __global__ __launch_bounds__(TPB) void k_modulus(int64_t ne, const double *__res
     if (E32) E32[e] = (float)v;
 }
 
+// Multiresolution moduli: E[8e + s] of sub-element s = sy + 2 sx + 4 sz of
+// element e, from densities x on the density mesh (2nelx x 2nely x 2nelz,
+// y fastest) (Implementation/GlobalStiffMatrix.m:19-62).
+__global__ __launch_bounds__(TPB) void k_modulus_mr(int nelx, int nely, int nelz, const double *__restrict__ x,
+                                                    double E0, double emin, double p, double *__restrict__ E64,
+                                                    float *__restrict__ E32) {
+    const int64_t ne = (int64_t)nelx * nely * nelz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= ne * 8) return;
+    const int64_t e = t >> 3;
+    const int s = (int)(t & 7);
+    const int ez = (int)(e / ((int64_t)nelx * nely)), ex = (int)((e / nely) % nelx), ey = (int)(e % nely);
+    const int sy = s & 1, sx = (s >> 1) & 1, sz = s >> 2;
+    const int64_t NX = 2 * (int64_t)nelx, NY = 2 * (int64_t)nely;
+    const int64_t v = (2 * (int64_t)ez + sz) * NX * NY + (2 * (int64_t)ex + sx) * NY + (2 * (int64_t)ey + sy);
+    const double val = emin + (E0 - emin) * pow(x[v], p);
+    E64[t] = val;
+    if (E32) E32[t] = (float)val;
+}
+
 // Jacobi at the fine level: invD = 1 / ((1/omega) diag(K)), diag(K) gathered
 // element by element (the recipe of Implementation/WOK/DiagPrec.m).
-__global__ __launch_bounds__(TPB) void k_diag_fine(int nelx, int nely, int nelz, const double *__restrict__ E,
+__global__ __launch_bounds__(TPB) void k_diag_fine(int nelx, int nely, int nelz, int nsub, const double *__restrict__ E,
                                                    const double *__restrict__ K0, const uint8_t *__restrict__ mask,
                                                    double omega, double *__restrict__ inv64, float *__restrict__ inv32) {
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
@@ -675,9 +811,10 @@ __global__ __launch_bounds__(TPB) void k_diag_fine(int nelx, int nely, int nelz,
             for (int oy = 0; oy < 2; ++oy) {
                 const int ex = ix - 1 + ox, ey = iy - 1 + oy, ez = iz - 1 + oz;
                 if (ex < 0 || ex >= nelx || ey < 0 || ey >= nely || ez < 0 || ez >= nelz) continue;
-                const double Ee = E[(int64_t)ez * nelx * nely + (int64_t)ex * nely + ey];
+                const int64_t e = (int64_t)ez * nelx * nely + (int64_t)ex * nely + ey;
                 const int ai = (1 - oy) + 2 * (1 - ox) + 4 * (1 - oz);
-                for (int c = 0; c < 3; ++c) dg[c] += Ee * K0[(3 * ai + c) * 24 + 3 * ai + c];
+                for (int s = 0; s < nsub; ++s)
+                    for (int c = 0; c < 3; ++c) dg[c] += E[nsub * e + s] * K0[576 * s + (3 * ai + c) * 24 + 3 * ai + c];
             }
     for (int c = 0; c < 3; ++c) {
         const int64_t dof = 3 * i + c;
@@ -710,9 +847,9 @@ __device__ inline double tri_w(int off) {  // 1-D trilinear weight of a fine nod
 // fine free-DOF mask, T_k the trilinear interpolation of the coarse element
 // onto child k. One thread per (coarse element, 3x3 output block); output
 // structure-of-arrays Ahat[(row*24 + col)*ne + E].
-__global__ __launch_bounds__(TPB) void k_galerkin_fine(int cnelx, int cnely, int cnelz, const double *__restrict__ E,
-                                                       const double *__restrict__ K0, const uint8_t *__restrict__ fmask,
-                                                       double *__restrict__ Ahat) {
+__global__ __launch_bounds__(TPB) void k_galerkin_fine(int cnelx, int cnely, int cnelz, int nsub,
+                                                       const double *__restrict__ E, const double *__restrict__ K0,
+                                                       const uint8_t *__restrict__ fmask, double *__restrict__ Ahat) {
     const int64_t cne = (int64_t)cnelx * cnely * cnelz;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= cne * 64) return;
@@ -727,7 +864,10 @@ __global__ __launch_bounds__(TPB) void k_galerkin_fine(int cnelx, int cnely, int
     double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int k = 0; k < 8; ++k) {
         const int ky = k & 1, kx = (k >> 1) & 1, kz = k >> 2;
-        const double Ek = E[(int64_t)(2 * Ez + kz) * fnelx * fnely + (int64_t)(2 * Ex + kx) * fnely + (2 * Ey + ky)];
+        const int64_t ek = (int64_t)(2 * Ez + kz) * fnelx * fnely + (int64_t)(2 * Ex + kx) * fnely + (2 * Ey + ky);
+        for (int s = 0; s < nsub; ++s) {
+        const double Ek = E[nsub * ek + s];
+        const double *Ks = K0 + 576 * s;
 #pragma unroll
         for (int an = 0; an < 8; ++an) {
             const int ay = an & 1, ax = (an >> 1) & 1, az = an >> 2;
@@ -747,8 +887,9 @@ __global__ __launch_bounds__(TPB) void k_galerkin_fine(int cnelx, int cnely, int
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) R[c][d] += ma[c] * mb[d] * K0[(3 * an + c) * 24 + 3 * bn + d];
+                    for (int d = 0; d < 3; ++d) R[c][d] += ma[c] * mb[d] * Ks[(3 * an + c) * 24 + 3 * bn + d];
             }
+        }
         }
     }
     for (int c = 0; c < 3; ++c)
@@ -909,6 +1050,43 @@ __global__ __launch_bounds__(TPB) void k_grad0(int nelx, int nely, int nelz, con
         s += ue[i] * t;
     }
     g0[e] = -p * pow(x[e], p - 1.0) * (E0 - emin) * s;
+}
+
+// Multiresolution g0 on the density mesh: for each sub-element v = (e, s),
+// g0_v = -p x_v^(p-1) (E0 - emin) u_e' K_s u_e (Implementation/Grad.m:24-49,
+// mr.interp = false); freedens masks displacement elements (str.freeDensG).
+__global__ __launch_bounds__(TPB) void k_grad0_mr(int nelx, int nely, int nelz, const double *__restrict__ u,
+                                                  const double *__restrict__ xd, const uint8_t *__restrict__ freedens,
+                                                  const double *__restrict__ Ks, double p, double E0, double emin,
+                                                  double *__restrict__ g0) {
+    const int64_t ne = (int64_t)nelx * nely * nelz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= ne * 8) return;
+    const int64_t e = t >> 3;
+    const int s = (int)(t & 7);
+    const int nx = nelx + 1, ny = nely + 1;
+    const int ez = (int)(e / ((int64_t)nelx * nely)), ex = (int)((e / nely) % nelx), ey = (int)(e % nely);
+    const int sy = s & 1, sx = (s >> 1) & 1, sz = s >> 2;
+    const int64_t NX = 2 * (int64_t)nelx, NY = 2 * (int64_t)nely;
+    const int64_t v = (2 * (int64_t)ez + sz) * NX * NY + (2 * (int64_t)ex + sx) * NY + (2 * (int64_t)ey + sy);
+    if (freedens && !freedens[e]) { g0[v] = 0.0; return; }
+    double ue[24];
+#pragma unroll
+    for (int an = 0; an < 8; ++an) {
+        const int dy = an & 1, dx = (an >> 1) & 1, dz = an >> 2;
+        const int64_t nd = (int64_t)(ez + dz) * nx * ny + (int64_t)(ex + dx) * ny + (ey + dy);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * an + c] = u[3 * nd + c];
+    }
+    const double *K = Ks + 576 * s;
+    double q = 0.0;
+    for (int i = 0; i < 24; ++i) {
+        double tt = 0.0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) tt += K[i * 24 + j] * ue[j];
+        q += ue[i] * tt;
+    }
+    g0[v] = -p * pow(xd[v], p - 1.0) * (E0 - emin) * q;
 }
 
 // Filter row sums w and weighted neighbourhood products on the element grid.

@@ -22,6 +22,7 @@ namespace {
 
 constexpr int MAXL = 12;
 constexpr int GRAPH_ITERS = 4;
+constexpr int64_t FUSED_LEVEL_MAXNODES = 100000;
 thread_local std::string g_setup_err;
 
 struct DevLevel {
@@ -48,7 +49,9 @@ struct tmg_solver {
     int L = 0;
     DevLevel lev[MAXL];
     double ke[576];
-    double *K0_64 = nullptr;
+    int mr = 1;                       // multiresolution factor n (1 = off; n = 2: 8 density sub-elements)
+    int nsub = 1;                     // n^3
+    double *K0_64 = nullptr;          // Ke, or the n^3 sub-element integrands (576 each)
     float *K0_32 = nullptr;
     double *E64 = nullptr;
     float *E32 = nullptr;
@@ -71,8 +74,9 @@ struct tmg_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;
-    bool ebe_tiled = true;            // LDS-staged fine-level apply (ny <= EBE_TILE_MAXNY)
-    bool fused = true;                // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
+    bool ebe_tiled = false;           // LDS-staged fine-level apply (ny <= EBE_TILE_MAXNY)
+    bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
+    bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
     // externally owned resources (a server slot): device memory comes from a
@@ -170,10 +174,34 @@ template <typename T, typename TS, int SRC, int EPI>
 void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
     const DevLevel &lv = h->lev[0];
     dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + h->zchunk - 1) / h->zchunk);
+#define EBE_ARGS a.nelx, a.nely, a.nelz, a.zchunk, a.x, a.E, a.K0, a.mask, a.b, a.invD, a.r, a.ec, a.cnx, \
+                 a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done
+    if constexpr (SRC == SRC_X) {
+        if (h->nsub == 8) {
+            hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 8>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
+            return;
+        }
+    }
     if (lv.ny <= EBE_TILE_MAXNY && h->ebe_tiled)
-        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, true>), grid, dim3(TPB), 0, h->st, a);
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, true, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
     else
-        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false>), grid, dim3(TPB), 0, h->st, a);
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
+#undef EBE_ARGS
+}
+
+#define ST_ARGS(a) (a).nx, (a).ny, (a).nz, (a).x, (a).S, (a).mask, (a).b, (a).invD, (a).ec, (a).cnx, (a).cny, (a).out, \
+                   (a).partial, (a).done
+
+template <typename T>
+StArgs<T> st_args(tmg_solver *h, int l) {
+    const DevLevel &lv = h->lev[l];
+    StArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.nx = lv.nx; a.ny = lv.ny; a.nz = lv.nz;
+    a.S = Sof<T>(lv); a.mask = lv.mask; a.invD = invof<T>(lv);
+    a.cnx = l + 1 < h->L ? h->lev[l + 1].nx : 1;
+    a.cny = l + 1 < h->L ? h->lev[l + 1].ny : 1;
+    return a;
 }
 
 template <typename T, int EPI>
@@ -184,17 +212,17 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
         a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
         launch_ebe<T, T, SRC_X, EPI>(h, a);
     } else {
-        hipLaunchKernelGGL((k_stencil<T, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, x,
-                           Sof<T>(lv), lv.mask, b, invof<T>(lv), out, partial, done);
+        StArgs<T> a = st_args<T>(h, l);
+        a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
+        hipLaunchKernelGGL((k_stencil<T, SRC_X, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
     }
 }
 
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
-    hipLaunchKernelGGL((k_gather<T>), dim3(nblk(h->nc)), dim3(TPB), 0, h->st, h->nc, lc.d_free, b, h->cb, done);
-    hipLaunchKernelGGL(k_gemv, dim3((unsigned)((h->nc + 3) / 4)), dim3(TPB), 0, h->st, h->nc, h->Ainv, h->cb, h->cx, done);
-    hipLaunchKernelGGL((k_scatter_full<T>), dim3(nblk(lc.ndof)), dim3(TPB), 0, h->st, lc.ndof, lc.d_cidx, h->cx, x, done);
+    hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)((h->nc + 3) / 4)), dim3(TPB), 0, h->st, h->nc, h->Ainv,
+                       lc.d_free, b, x, done);
 }
 
 // One V(nu1,nu2) cycle from a zero initial guess at level l
@@ -205,6 +233,26 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done) {
     DevLevel &lc = h->lev[l + 1];
     T *xa = (T *)lv.x, *xb = (T *)lv.t;
     const int nu1 = h->mg.smoothiter1, nu2 = h->mg.smoothiter2;
+    // small stencil levels are launch-bound: fold the sweeps into the applies there
+    if (l > 0 && nu1 == 1 && nu2 == 1 && h->fused_coarse && lv.nn <= FUSED_LEVEL_MAXNODES) {
+        // stencil level, V(1,1): rh = b - A (M\b); then s = w + M\(b - A w), w = M\b + P eH
+        StArgs<T> a = st_args<T>(h, l);
+        a.b = b; a.out = xb; a.done = done;
+        hipLaunchKernelGGL((k_stencil<T, SRC_JZ, EPI_RESID>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
+        hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
+                           lc.nz, xb, lc.mask, (T *)lc.b, done);
+        T *ec;
+        if (l + 1 == h->L - 1) {
+            coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);
+            ec = (T *)lc.x;
+        } else {
+            ec = vcycle<T>(h, l + 1, (const T *)lc.b, done);
+        }
+        StArgs<T> c = st_args<T>(h, l);
+        c.b = b; c.ec = ec; c.out = xa; c.done = done;
+        hipLaunchKernelGGL((k_stencil<T, SRC_POST, EPI_JACOBI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(c));
+        return xa;
+    }
     if (nu1 >= 1) {
         // first sweep from x0 = 0: x = M \ b (Implementation/Smooth.m:17 with x = 0)
         hipLaunchKernelGGL((k_jacobi_zero<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, b, invof<T>(lv), xa, done);
@@ -263,7 +311,7 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
     return (T *)l0.x;
 }
 
-bool fused_ok(const tmg_solver *h) { return h->fused && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1; }
+bool fused_ok(const tmg_solver *h) { return h->fused && h->nsub == 1 && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1; }
 
 // Preconditioner application s = B r for the PCG; r is the FP64 residual.
 // Returns s as a typed pointer through one of the two out-params.
@@ -347,6 +395,7 @@ int solve_begin(tmg_solver *h) {
     // d = 0 so that the first direction update is exact
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd2);
+
     // r = f - A x0 (Implementation/SolveMG.m:26-27)
     apply<double, EPI_RESID>(h, 0, h->px, h->pf, h->pr, nullptr, nullptr);
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), h->npartial);
@@ -442,6 +491,15 @@ int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
     TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * vs)));
     TRY(dalloc(h, (char **)&lv.x, (int64_t)(lv.ndof * vs)));
     TRY(dalloc(h, (char **)&lv.t, (int64_t)(lv.ndof * vs)));
+    // level vectors start at zero: fixed DOFs of a level vector are never
+    // written afterwards except with zero (the coarse solve skips them)
+    if (h->prec == TMG_MIXED) {
+        for (void *v : {lv.b, lv.x, lv.t})
+            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0f, (float *)v);
+    } else {
+        for (void *v : {lv.b, lv.x, lv.t})
+            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0, (double *)v);
+    }
     TRY(dalloc(h, &lv.inv64, lv.ndof));
     if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.inv32, lv.ndof));
     if (l > 0) {
@@ -482,13 +540,19 @@ void destroy_all(tmg_solver *h) {
 // Element stiffness by exact 2x2x2 Gauss integration of B'DB on the box
 // (equal to the closed form of Implementation/ElemStiffMatrix.m, local DOF
 // order 3*(dy + 2 dx + 4 dz) + c).
-void element_stiffness(double el, double eh, double ew, double nu, double *k) {
+// Integral of B'DB over the sub-box [lo, lo + h]^3 of the reference cube
+// [-1, 1]^3 (2-point Gauss per direction, exact for Q1 on boxes); the whole
+// cube (lo = -1, h = 2) gives Ke, the 2x2x2 sub-boxes give the
+// multiresolution integrands of Implementation/ElemStiffMatrixGQMR.m.
+void element_stiffness_box(double el, double eh, double ew, double nu, const double lo[3], double hb, double *k) {
     const double a = 0.5 * el, b = 0.5 * eh, c = 0.5 * ew;
     const double g = 1.0 / std::sqrt(3.0);
     const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
     std::fill(k, k + 576, 0.0);
+    const double s = 0.5 * hb;  // half-width of the sub-box in reference coordinates
     for (int q = 0; q < 8; ++q) {
-        const double xi = (q & 2 ? g : -g), et = (q & 1 ? g : -g), ze = (q & 4 ? g : -g);
+        const double xi = lo[0] + s + s * (q & 2 ? g : -g), et = lo[1] + s + s * (q & 1 ? g : -g),
+                     ze = lo[2] + s + s * (q & 4 ? g : -g);
         double dN[8][3];
         for (int an = 0; an < 8; ++an) {
             const double sy = (an & 1) ? 1 : -1, sx = (an & 2) ? 1 : -1, sz = (an & 4) ? 1 : -1;
@@ -496,7 +560,7 @@ void element_stiffness(double el, double eh, double ew, double nu, double *k) {
             dN[an][1] = sy * (1 + sx * xi) * (1 + sz * ze) / (8 * b);
             dN[an][2] = sz * (1 + sx * xi) * (1 + sy * et) / (8 * c);
         }
-        const double w = a * b * c;
+        const double w = a * b * c * s * s * s;
         for (int i = 0; i < 8; ++i)
             for (int j = 0; j < 8; ++j) {
                 const double *gi = dN[i], *gj = dN[j];
@@ -514,19 +578,28 @@ void element_stiffness(double el, double eh, double ew, double nu, double *k) {
         for (int j = i + 1; j < 24; ++j) k[j * 24 + i] = k[i * 24 + j];
 }
 
+void element_stiffness(double el, double eh, double ew, double nu, double *k) {
+    const double lo[3] = {-1.0, -1.0, -1.0};
+    element_stiffness_box(el, eh, ew, nu, lo, 2.0, k);
+}
+
 int refresh(tmg_solver *h) {
     DevLevel &l0 = h->lev[0];
     const double om = h->mg.omega;
-    hipLaunchKernelGGL(k_modulus, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.ne, h->xdens, h->E0, h->emin, h->penal,
-                       h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
-    hipLaunchKernelGGL(k_diag_fine, dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->E64, h->K0_64,
-                       l0.mask, om, l0.inv64, h->prec == TMG_MIXED ? l0.inv32 : nullptr);
+    if (h->nsub == 1)
+        hipLaunchKernelGGL(k_modulus, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.ne, h->xdens, h->E0, h->emin, h->penal,
+                           h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
+    else
+        hipLaunchKernelGGL(k_modulus_mr, dim3(nblk(l0.ne * 8)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->xdens,
+                           h->E0, h->emin, h->penal, h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
+    hipLaunchKernelGGL(k_diag_fine, dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->nsub, h->E64,
+                       h->K0_64, l0.mask, om, l0.inv64, h->prec == TMG_MIXED ? l0.inv32 : nullptr);
     for (int l = 1; l < h->L; ++l) {
         DevLevel &lv = h->lev[l];
         double *dst = h->Ahat[(l - 1) & 1];
         if (l == 1) {
             hipLaunchKernelGGL(k_galerkin_fine, dim3(nblk(lv.ne * 64)), dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz,
-                               h->E64, h->K0_64, l0.mask, dst);
+                               h->nsub, h->E64, h->K0_64, l0.mask, dst);
         } else {
             hipLaunchKernelGGL(k_galerkin_coarse, dim3(nblk(lv.ne * 64)), dim3(TPB), 0, h->st, lv.nelx, lv.nely,
                                lv.nelz, h->Ahat[l & 1], h->lev[l - 1].mask, dst);
@@ -616,7 +689,8 @@ struct ExtRes {
     hipMemPool_t pool = nullptr;
 };
 
-int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, const ExtRes *ext, tmg_solver **out) {
+int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, const ExtRes *ext, tmg_solver **out,
+               int mr_n = 1) {
     g_setup_err.clear();
     if (!prob || !mg || !out) { g_setup_err = "null argument"; return TMG_ERR_ARG; }
     *out = nullptr;
@@ -641,6 +715,10 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         return TMG_ERR_ARG;
     }
     if (precision != TMG_FP64 && precision != TMG_MIXED) { g_setup_err = "unknown precision"; return TMG_ERR_ARG; }
+    if (mr_n != 1 && mr_n != 2) {
+        g_setup_err = "multiresolution is supported with n = 2 (mr.n = mr.d = 2) only";
+        return TMG_ERR_UNSUPPORTED;
+    }
 
     tmg_solver *h = new tmg_solver();
     h->nelx = prob->nelx; h->nely = prob->nely; h->nelz = prob->nelz;
@@ -649,9 +727,13 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->mg = *mg;
     h->prec = precision;
     h->L = mg->ngrids;
+    h->mr = mr_n;
+    h->nsub = mr_n * mr_n * mr_n;
+    if (h->nsub > 1) h->fused = false;
     if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
     if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
-    if (const char *ev = getenv("TMG_UNFUSED")) h->fused = ev[0] == '0';
+    if (const char *ev = getenv("TMG_FUSED_FINE")) h->fused = ev[0] == '1';
+    if (const char *ev = getenv("TMG_UNFUSED_COARSE")) h->fused_coarse = ev[0] == '0';
     if (ext) {
         h->st = ext->st;
         h->rb = ext->rb;
@@ -710,11 +792,24 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->nc = h->lev[h->L - 1].nfree;
     const DevLevel &l0 = h->lev[0];
     element_stiffness(h->el, h->eh, h->ew, h->nu, h->ke);
-    float ke32[576];
-    for (int i = 0; i < 576; ++i) ke32[i] = (float)h->ke[i];
+    // the operator's element matrices: Ke, or the 8 sub-element integrands
+    // (order z, x, y with y fastest, Implementation/ElemStiffMatrixGQMR.m:54-56)
+    std::vector<double> kop(576 * h->nsub);
+    if (h->nsub == 1) {
+        std::copy(h->ke, h->ke + 576, kop.begin());
+    } else {
+        for (int sidx = 0; sidx < 8; ++sidx) {
+            const int sy = sidx & 1, sx = (sidx >> 1) & 1, sz = sidx >> 2;
+            const double lo[3] = {-1.0 + sx, -1.0 + sy, -1.0 + sz};
+            element_stiffness_box(h->el, h->eh, h->ew, h->nu, lo, 1.0, kop.data() + 576 * sidx);
+        }
+    }
+    std::vector<float> ke32(kop.size());
+    for (size_t i = 0; i < kop.size(); ++i) ke32[i] = (float)kop[i];
     int rc = 0;
-    if ((rc = dalloc(h, &h->K0_64, 576)) || (rc = dalloc(h, &h->K0_32, 576)) || (rc = dalloc(h, &h->E64, l0.ne)) ||
-        (rc = dalloc(h, &h->E32, l0.ne)) || (rc = dalloc(h, &h->xdens, l0.ne)) ||
+    const int64_t nem = l0.ne * h->nsub;
+    if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
+        (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
@@ -736,8 +831,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->err = "hipHostMalloc failed";
         return fail(TMG_ERR_OOM);
     }
-    if (hipMemcpyAsync(h->K0_64, h->ke, sizeof(h->ke), hipMemcpyHostToDevice, h->st) != hipSuccess ||
-        hipMemcpyAsync(h->K0_32, ke32, sizeof(ke32), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+    if (hipMemcpyAsync(h->K0_64, kop.data(), sizeof(double) * kop.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->K0_32, ke32.data(), sizeof(float) * ke32.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
@@ -761,17 +856,22 @@ int tmg_setup(const tmg_problem *prob, const tmg_mg_params *mg, int precision, t
     return setup_impl(prob, mg, precision, nullptr, out);
 }
 
+int tmg_setup_mr(const tmg_problem *prob, const tmg_mg_params *mg, int precision, int mr_n, tmg_solver **out) {
+    return setup_impl(prob, mg, precision, nullptr, out, mr_n);
+}
+
 int tmg_update_density_device(tmg_solver *h, const double *d_x) {
     if (!h || !d_x) return TMG_ERR_ARG;
-    HIPX(hipMemcpyAsync(h->xdens, d_x, sizeof(double) * h->lev[0].ne, hipMemcpyDeviceToDevice, h->st));
+    HIPX(hipMemcpyAsync(h->xdens, d_x, sizeof(double) * h->lev[0].ne * h->nsub, hipMemcpyDeviceToDevice, h->st));
     return refresh(h);
 }
 
 int tmg_update_density(tmg_solver *h, const double *x) {
     if (!h || !x) return TMG_ERR_ARG;
-    for (int64_t e = 0; e < h->lev[0].ne; ++e)
+    const int64_t n = h->lev[0].ne * h->nsub;
+    for (int64_t e = 0; e < n; ++e)
         if (!(x[e] >= 0.0) || !std::isfinite(x[e])) return set_err(h, TMG_ERR_ARG, "densities must be finite and >= 0");
-    HIPX(hipMemcpyAsync(h->xdens, x, sizeof(double) * h->lev[0].ne, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(h->xdens, x, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
     return refresh(h);
 }
 
@@ -959,8 +1059,9 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
             else direction_apply<double>(h, (const double *)l0.x, h->pd, h->pd2, nullptr);
         }
     };
-    // level vectors need finite values for the sweeps
-    for (int l = 0; l < 2 && l < h->L; ++l) {
+    // level vectors need finite values for the sweeps (not the coarsest
+    // level, whose fixed DOFs must stay zero)
+    for (int l = 0; l < 2 && l < h->L - 1; ++l) {
         DevLevel &lv = h->lev[l];
         if (mixed) {
             hipLaunchKernelGGL((k_fill<float>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 1.0f, (float *)lv.x);
@@ -1154,23 +1255,29 @@ int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *fre
              double *Fgrad) {
     if (!h || !u || !x || !Fgrad) return TMG_ERR_ARG;
     const DevLevel &l0 = h->lev[0];
-    if (filt && (filt->nelx != l0.nelx || filt->nely != l0.nely || filt->nelz != l0.nelz))
-        return set_err(h, TMG_ERR_ARG, "filter grid does not match the problem grid");
+    const int m = h->mr;  // gradient lives on the density mesh (m = 2 with multiresolution)
+    if (filt && (filt->nelx != m * l0.nelx || filt->nely != m * l0.nely || filt->nelz != m * l0.nelz))
+        return set_err(h, TMG_ERR_ARG, "filter grid does not match the density grid");
+    const int64_t nd = l0.ne * h->nsub;
     double *du = nullptr, *dxe = nullptr, *dg = nullptr, *dF = nullptr;
     uint8_t *dfd = nullptr;
     TRY(dalloc(h, &du, l0.ndof));
-    TRY(dalloc(h, &dxe, l0.ne));
-    TRY(dalloc(h, &dg, l0.ne));
-    TRY(dalloc(h, &dF, l0.ne));
+    TRY(dalloc(h, &dxe, nd));
+    TRY(dalloc(h, &dg, nd));
+    TRY(dalloc(h, &dF, nd));
     if (freedens) TRY(dalloc(h, &dfd, l0.ne));
     HIPX(hipMemcpyAsync(du, u, sizeof(double) * l0.ndof, hipMemcpyHostToDevice, h->st));
-    HIPX(hipMemcpyAsync(dxe, x, sizeof(double) * l0.ne, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(dxe, x, sizeof(double) * nd, hipMemcpyHostToDevice, h->st));
     if (freedens) HIPX(hipMemcpyAsync(dfd, freedens, l0.ne, hipMemcpyHostToDevice, h->st));
-    hipLaunchKernelGGL(k_grad0, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
-                       h->K0_64, h->penal, h->E0, h->emin, dg);
+    if (h->nsub == 1)
+        hipLaunchKernelGGL(k_grad0, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
+                           h->K0_64, h->penal, h->E0, h->emin, dg);
+    else
+        hipLaunchKernelGGL(k_grad0_mr, dim3(nblk(nd)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
+                           h->K0_64, h->penal, h->E0, h->emin, dg);
     int rc = 0;
     if (filt) rc = filter_apply_dev(const_cast<tmg_filter *>(filt), dg, 1, dF, h->st);
-    if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * l0.ne, hipMemcpyDeviceToHost, h->st));
+    if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * nd, hipMemcpyDeviceToHost, h->st));
     HIPX(hipStreamSynchronize(h->st));
     dfree(h, du); dfree(h, dxe); dfree(h, dg); dfree(h, dF); dfree(h, dfd);
     return rc ? set_err(h, rc, "grad kernel launch failed") : TMG_OK;

@@ -62,6 +62,7 @@ _pu8 = C.POINTER(C.c_uint8)
 
 _SIGS = {
     "tmg_setup": (_i, [C.POINTER(Problem_t), C.POINTER(MgParams_t), _i, C.POINTER(_p)]),
+    "tmg_setup_mr": (_i, [C.POINTER(Problem_t), C.POINTER(MgParams_t), _i, _i, C.POINTER(_p)]),
     "tmg_update_density": (_i, [_p, _pd]),
     "tmg_update_density_device": (_i, [_p, _p]),
     "tmg_solve": (_i, [_p, _pd, _pd, _pd, _pi, _pd]),
@@ -79,6 +80,7 @@ _SIGS = {
     "tmg_filter_rowsum": (_i, [_p, _pd]),
     "tmg_filter_destroy": (None, [_p]),
     "tmg_time_kernel": (_i, [_p, _i, _i, _pd]),
+    "tmg_lp_volume": (_i, [C.c_int64, _pd, _pd, _pd, _pd, _d, _i, _pd, _pd]),
     "tmg_server_create": (_i, [_d, _i, C.POINTER(_p)]),
     "tmg_server_submit": (_i, [_p, C.POINTER(Problem_t), C.POINTER(MgParams_t), _i, _pd, _pd, _pd, _d, _pi64]),
     "tmg_server_wait": (_i, [_p, C.c_int64, _d, _pi]),

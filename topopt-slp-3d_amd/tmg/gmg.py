@@ -51,8 +51,10 @@ class Solver:
     """Device-resident GMG-PCG hierarchy for one structured-grid problem."""
 
     def __init__(self, prob, ngrids, precision="fp64", omega=0.6, smoothiter1=1, smoothiter2=1,
-                 tol=1e-8, maxiter=5000, p=3.0, emin=1e-9, E0=None):
+                 tol=1e-8, maxiter=5000, p=3.0, emin=1e-9, E0=None, mr=1):
         self.prob = prob
+        self.mr = int(mr)
+        self.ndens = prob.nelem * self.mr ** 3   # densities / gradient entries (density mesh)
         self.precision = _PREC[precision]
         fd = np.ascontiguousarray(prob.freedofs, dtype=np.int64)
         self._fd = fd
@@ -61,7 +63,10 @@ class Solver:
                             fd.ctypes.data_as(C.POINTER(C.c_int64)), fd.size)
         mg = _lib.MgParams_t(ngrids, 0, 0, omega, smoothiter1, smoothiter2, tol, maxiter, 0, 60000)
         h = C.c_void_p()
-        rc = lib().tmg_setup(C.byref(pr), C.byref(mg), self.precision, C.byref(h))
+        if self.mr == 1:
+            rc = lib().tmg_setup(C.byref(pr), C.byref(mg), self.precision, C.byref(h))
+        else:
+            rc = lib().tmg_setup_mr(C.byref(pr), C.byref(mg), self.precision, self.mr, C.byref(h))
         if rc != 0:
             raise TmgError(rc, _lib.last_error(None))
         self.h = h
@@ -90,7 +95,7 @@ class Solver:
         if hasattr(x, "data_ptr"):  # device tensor (float64, n_el)
             return self._chk(lib().tmg_update_density_device(self.h, C.c_void_p(x.data_ptr())))
         x = _f64(x)
-        if x.size != self.prob.nelem:
+        if x.size != self.ndens:
             raise ValueError("density vector has the wrong length")
         return self._chk(lib().tmg_update_density(self.h, _dp(x)))
 
@@ -172,7 +177,7 @@ class Solver:
 
     def grad(self, u_full, x, freedens=None, filt=None):
         u_full, x = _f64(u_full), _f64(x)
-        g = np.zeros(self.prob.nelem)
+        g = np.zeros(self.ndens)
         fd = None
         if freedens is not None:
             fdm = np.ascontiguousarray(freedens, dtype=np.uint8)
