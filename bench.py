@@ -37,7 +37,25 @@ def parse():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--workload", default="solve", choices=["solve", "slp"],
+                    help="solve: MG-PCG solves + serving stream (default); slp: the multiresolution SLP loop")
+    ap.add_argument("--slp-iters", type=int, default=30)
+    ap.add_argument("--slp-grid", default="240x120x120", help="displacement mesh of the SLP workload")
     return ap.parse_args()
+
+
+def profile_traffic(name):
+    """HBM bytes per launch recorded in a committed rocprofv3 counter summary
+    (line 'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def cpu_baseline():
@@ -77,12 +95,53 @@ def cpu_baseline():
     }
 
 
+def slp_workload(args):
+    """The multiresolution SLP loop: 240x120x120 displacement elements
+    (10.5M DOFs), 2x2x2 density sub-elements (27.6M design variables),
+    volfrac 0.3, filter radius 1.5, 4-level GMG, mixed precision; solve,
+    Galerkin refresh, filter and gradient on the GPU, the LP on the host."""
+    import torch
+
+    from tmg import problem as P
+    from tmg import slp
+    torch.cuda.init()
+    nelx, nely, nelz = (int(v) for v in args.slp_grid.split("x"))
+    prob = P.cantilever(nelx, nely, nelz)
+    log(f"SLP: {nelx}x{nely}x{nelz} MR 2, {prob.nfree} free DOFs, {8 * prob.nelem} design variables")
+    t0 = time.perf_counter()
+    res = slp.StrSLP(prob, 0.3, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, ngrids=4, precision="mixed", mr=2,
+                     slp=slp.SLPParams(maxiter=args.slp_iters), log=log)
+    wall = time.perf_counter() - t0
+    nsolve = res.ns
+    line = {
+        "metric": f"SLP iterations/s (MR 2x2x2, {nelx}x{nely}x{nelz}, mixed precision GMG-PCG, host LP)",
+        "value": round(res.iter / wall, 4), "unit": "accepted SLP iterations/s", "n_gpus": 1,
+        "steps": res.iter, "warmup": 0, "ms_per_step": round(1e3 * wall / max(res.iter, 1), 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (PCG, LP, gradient) + f32 (V-cycle)", "data": "synthetic",
+        "config": {
+            "workload": f"cantilever {nelx}x{nely}x{nelz} Q1 displacement mesh, multiresolution 2x2x2 density/design "
+                        f"sub-elements ({2 * nelx}x{2 * nely}x{2 * nelz}), volfrac 0.3, linear filter rmin 1.5, "
+                        f"4-level GMG V(1,1), tol 1e-8",
+            "n_free_dofs": prob.nfree, "design_variables": 8 * prob.nelem,
+            "iterations": res.iter, "rejected": res.itrej, "solves": nsolve, "pcg_iterations": res.pcgit,
+            "F": res.F, "vol": res.vol, "opstop": res.opstop, "wall_s": round(wall, 3),
+            "time_s": {kk: round(v, 3) for kk, v in res.time.items()},
+            "F_history": [float(f"{h['F']:.8g}") for h in res.history],
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "slp":
+        slp_workload(args)
+        return
     import numpy as np
     import torch
 
@@ -158,19 +217,41 @@ def main():
         "vcycle_fp64_ms": sf.time_kernel(3, max(reps // 5, 3)),
     }
     log("kernels:", json.dumps(k))
-    # fine-level Jacobi sweep in FP32: read x, b, invD; write x; read E
-    bytes_jac32 = 4 * 4 * n_dof + 4 * n_el
+    # Dominant solve kernel: the FP64 fine-level operator apply q = K(rho) d
+    # with the PCG dot d'q folded in (once per iteration in both modes).
+    # Algorithmic bytes: read d, write q (8 B x n_dof each), read E (8 B x n_el).
     flops_apply = 1152 * n_el
-    t = k["fine_jacobi_fp32_ms"] * 1e-3
-    achieved = bytes_jac32 / t / 1e9
+    bytes_apply64 = 8 * 2 * n_dof + 8 * n_el
+    t = k["fine_apply_fp64_ms"] * 1e-3
+    achieved = bytes_apply64 / t / 1e9
     roofline = {
-        "bound": "hbm", "kernel": "k_ebe<float,JACOBI> (fine-level fused apply + damped Jacobi, FP32)",
+        "bound": "hbm", "kernel": "k_ebe<double,DOT> (fine-level FP64 operator apply q = K d with d'q)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "bytes_per_launch": bytes_jac32, "avg_launch_ms": round(k["fine_jacobi_fp32_ms"], 5),
-        "valu_tflops": round(flops_apply / t / 1e12, 2), "valu_peak_tflops": FP32_VEC_PEAK_TFLOPS,
-        "valu_frac": round(flops_apply / t / 1e12 / FP32_VEC_PEAK_TFLOPS, 4),
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": profile_traffic("k_ebe_apply_fp64_cantilever192x96x96_hbm_traffic.txt"),
+        "bytes_per_launch": bytes_apply64, "avg_launch_ms": round(k["fine_apply_fp64_ms"], 5),
+        "valu_tflops": round(flops_apply / t / 1e12, 2), "valu_peak_tflops": FP64_VEC_PEAK_TFLOPS,
+        "valu_frac": round(flops_apply / t / 1e12 / FP64_VEC_PEAK_TFLOPS, 4),
     }
+    # the preconditioner's fine-level sweep (FP32): read x, b, invD, E; write x
+    bytes_jac32 = 4 * 4 * n_dof + 4 * n_el
+    t32 = k["fine_jacobi_fp32_ms"] * 1e-3
+    secondary = {
+        "kernel": "k_ebe<float,JACOBI> (fine-level fused apply + damped Jacobi sweep, FP32)",
+        "achieved_gbs": round(bytes_jac32 / t32 / 1e9, 1), "frac": round(bytes_jac32 / t32 / 1e9 / HBM_PEAK_GBS, 4),
+        "bytes_per_launch": bytes_jac32, "avg_launch_ms": round(k["fine_jacobi_fp32_ms"], 5),
+        "traffic": profile_traffic("k_ebe_jacobi_fp32_cantilever192x96x96_hbm_traffic.txt"),
+        "valu_tflops": round(flops_apply / t32 / 1e12, 2), "valu_peak_tflops": FP32_VEC_PEAK_TFLOPS,
+        "valu_frac": round(flops_apply / t32 / 1e12 / FP32_VEC_PEAK_TFLOPS, 4),
+    }
+    serving = None
+    if not args.no_serve:
+        from tmg import serve
+        log(f"serving: building {args.serve_requests} requests ...")
+        reqs = serve.stream_workload(args.serve_requests, seed=4)
+        serving = serve.benchmark(reqs, budget_bytes=200e9, max_active=16, cap_sample=32, load=0.8, log=log)
+        serving = {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in serving.items()}
+        log("serving:", json.dumps(serving))
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -198,6 +279,9 @@ def main():
             "mixed": {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in mixed.items()},
             "speedup_mixed_vs_fp64_time_to_solution": round(fp64["t_solve"] / mixed["t_solve"], 3),
             "kernels_ms": k,
+            "preconditioner_sweep": secondary,
+            "serving": serving,
+            "served_p99_ms": serving["p99_ms"] if serving else None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -60,12 +60,12 @@ typedef struct tmg_problem {
     int64_t nfree;
 } tmg_problem;
 
-/* The `mg` struct (Implementation/TopOpt3D.m:171-188). Only cycle = 0
- * (V-cycle), smoother = 0 (Jacobi) and opchol = 0 (coarse Cholesky) are
- * accepted, the configuration of the reference's default GMG path. */
+/* The `mg` struct (Implementation/TopOpt3D.m:171-188). cycle = 0 (V) or 1
+ * (W), smoother = 0 (Jacobi) and opchol = 0 (coarse Cholesky) are accepted;
+ * the FMV cycle and the SOR/SSOR smoothers return TMG_ERR_UNSUPPORTED. */
 typedef struct tmg_mg_params {
     int ngrids;          /* mg.ngrids                                             */
-    int cycle;           /* mg.cycle        (0 = V)                               */
+    int cycle;           /* mg.cycle        (0 = V, 1 = W)                        */
     int smoother;        /* mg.smoother     (0 = Jacobi)                          */
     double omega;        /* mg.omega                                              */
     int smoothiter1;     /* mg.smoothiter1                                        */
@@ -114,7 +114,9 @@ int tmg_solve(tmg_solver *h, const double *x0, const double *f, double *x_out, i
 int tmg_solve_device(tmg_solver *h, double *d_x, const double *d_f, int *it, double *relres);
 
 /* x = VCycle(A,b,zeros,niter1,niter2,ngrids,1,...) (Implementation/VCycle.m:1-45),
- * one V-cycle at the finest level on nfree host vectors. */
+ * or WCycle(...) (Implementation/WCycle.m:1-27) for a handle set up with
+ * mg->cycle = 1: one preconditioner cycle at the finest level on nfree host
+ * vectors. */
 int tmg_vcycle(tmg_solver *h, const double *b, double *x);
 
 /* x = Smooth(A{level},M{level},L,U,b,x0,niter,0) (Implementation/Smooth.m:1-24),
@@ -137,6 +139,12 @@ int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc);
  * (the freedofsc output of Prolongation.m). Level 1-based. */
 int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t *freedofs1);
 int tmg_ngrids(const tmg_solver *h);
+/* Mixed precision (no counterpart in the reference): iteration at which the
+ * last solve switched its preconditioner from the FP32 to the FP64 hierarchy
+ * because the residual stalled (0: it did not), and the number of such
+ * switches over the handle's lifetime. A switch restarts the conjugate
+ * directions (beta = 0) and leaves the stop rule unchanged. */
+int tmg_solve_stats(const tmg_solver *h, int *escalated_at, int *escalations);
 
 /* Fgrad = Grad(str,u,p,emin,k,x,Dofs,W,w,opfilter,mr,strDens)
  * (Implementation/Grad.m:18-62; with a multiresolution handle x, Fgrad and

@@ -53,6 +53,9 @@ def lib():
         L.or_vcycle.argtypes = [_p, _i, _p, _i, _i, _p]
         L.or_solve_mg.restype = _i
         L.or_solve_mg.argtypes = [_p, _p, _p, _i, _i, _d, _i, _p]
+        L.or_wcycle.argtypes = [_p, _i, _p, _p, _i, _i, _p]
+        L.or_solve_mg_cycle.restype = _i
+        L.or_solve_mg_cycle.argtypes = [_p, _p, _p, _i, _i, _i, _d, _i, _p]
         L.or_csr_nnz.restype = _i64
         L.or_csr_nnz.argtypes = [_p, _i, _i]
         L.or_csr_export.argtypes = [_p, _i, _i, _p, _p, _p]
@@ -168,11 +171,18 @@ class Oracle:
         lib().or_vcycle(self.h, level, _ptr(b), nu1, nu2, _ptr(x))
         return x
 
-    def solve(self, f, x0=None, nu1=1, nu2=1, tol=1e-8, maxiter=5000):
+    def wcycle(self, b, x0=None, nu1=1, nu2=1, level=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        lib().or_wcycle(self.h, level, _ptr(b), _ptr(x0a), nu1, nu2, _ptr(x))
+        return x
+
+    def solve(self, f, x0=None, nu1=1, nu2=1, tol=1e-8, maxiter=5000, cycle=0):
         f = np.ascontiguousarray(f, dtype=np.float64)
         x = np.zeros_like(f) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
         rr = np.zeros(1)
-        it = lib().or_solve_mg(self.h, _ptr(x), _ptr(f), nu1, nu2, tol, maxiter, _ptr(rr))
+        it = lib().or_solve_mg_cycle(self.h, _ptr(x), _ptr(f), cycle, nu1, nu2, tol, maxiter, _ptr(rr))
         return x, int(it), float(rr[0])
 
     def csr(self, which, level):

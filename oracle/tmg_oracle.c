@@ -678,8 +678,44 @@ void or_vcycle(const or_ctx *c, int nv, const double *b, int nu1, int nu2, doubl
     free(zero); free(rh); free(rH); free(eH); free(eh);
 }
 
+/* x = WCycle(A, b, x0, nu1, nu2, ..., nv) (Implementation/WCycle.m:1-27):
+ * the coarse correction recurses twice, the second time from the first
+ * result; the level above the coarsest solves directly once. x0 may be NULL
+ * (zero initial guess). */
+void or_wcycle(const or_ctx *c, int nv, const double *b, const double *x0, int nu1, int nu2, double *x) {
+    int64_t n = c->lev[nv].nfree, nc = c->lev[nv + 1].nfree;
+    double *zero = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double *rh = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *rH = (double *)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    double *eH = (double *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+    double *eh = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    or_smooth(c, nv, b, x0 ? x0 : zero, nu1, x);       /* WCycle.m:3 */
+    spmv(&c->A[nv], x, rh);                             /* WCycle.m:4 */
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) rh[i] = b[i] - rh[i];
+    spmv(&c->PT[nv], rh, rH);                           /* WCycle.m:5 */
+    if (nv == c->L - 2) {
+        coarse_solve(c, rH, eH);                        /* WCycle.m:15-16 */
+    } else {
+        double *e1 = (double *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+        or_wcycle(c, nv + 1, rH, NULL, nu1, nu2, e1);   /* WCycle.m:19 */
+        or_wcycle(c, nv + 1, rH, e1, nu1, nu2, eH);     /* WCycle.m:20 */
+        free(e1);
+    }
+    spmv(&c->P[nv], eH, eh);                            /* WCycle.m:22 */
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) x[i] = x[i] + eh[i];
+    or_smooth(c, nv, b, x, nu2, x);                     /* WCycle.m:24 */
+    free(zero); free(rh); free(rH); free(eH); free(eh);
+}
+
 int or_solve_mg(const or_ctx *c, double *x, const double *f, int nu1, int nu2,
                 double tol, int maxiter, double *relres) {
+    return or_solve_mg_cycle(c, x, f, 0, nu1, nu2, tol, maxiter, relres);
+}
+
+int or_solve_mg_cycle(const or_ctx *c, double *x, const double *f, int cycle, int nu1, int nu2,
+                      double tol, int maxiter, double *relres) {
     int64_t n = c->lev[0].nfree;
     size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
     double *r = (double *)malloc(bytes), *s = (double *)malloc(bytes);
@@ -692,8 +728,12 @@ int or_solve_mg(const or_ctx *c, double *x, const double *f, int nu1, int nu2,
     int it = 1;
     double gamma2 = 0.0;
     while (nr > tol && it <= maxiter) {                 /* SolveMG.m:30 */
-        if (c->L >= 2) or_vcycle(c, 0, r, nu1, nu2, s);
-        else memcpy(s, r, bytes);
+        if (c->L >= 2) {                                /* SolveMG.m:31-37 */
+            if (cycle == 1) or_wcycle(c, 0, r, NULL, nu1, nu2, s);
+            else or_vcycle(c, 0, r, nu1, nu2, s);
+        } else {
+            memcpy(s, r, bytes);
+        }
         double rho = ddot(n, r, s);                     /* SolveMG.m:38 */
         if (it == 1) memcpy(d, s, bytes);
         else {

@@ -63,6 +63,12 @@ void or_vcycle(const or_ctx *c, int level, const double *b, int nu1, int nu2, do
  * final recursive ||r||/||f||. */
 int or_solve_mg(const or_ctx *c, double *x, const double *f, int nu1, int nu2,
                 double tol, int maxiter, double *relres);
+/* x = WCycle(A, b, x0, nu1, nu2, ..., nv=level) (Implementation/WCycle.m:1-27);
+ * x0 may be NULL for a zero initial guess. */
+void or_wcycle(const or_ctx *c, int level, const double *b, const double *x0, int nu1, int nu2, double *x);
+/* SolveMG with the cycle switch of Implementation/SolveMG.m:31-37 (0 V, 1 W). */
+int or_solve_mg_cycle(const or_ctx *c, double *x, const double *f, int cycle, int nu1, int nu2,
+                      double tol, int maxiter, double *relres);
 
 /* Export a level's operator / prolongation as CSR (compact indices). */
 int64_t or_csr_nnz(const or_ctx *c, int which, int level); /* which: 0=A, 1=P */

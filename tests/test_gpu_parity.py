@@ -104,6 +104,59 @@ def test_solve_small(oracle_mod, precision, nelx, nely, nelz, L, seed):
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", [(16, 8, 8, 3, 12), (16, 8, 8, 4, 14)])
+def test_wcycle_matches_oracle(oracle_mod, precision, nelx, nely, nelz, L, seed):
+    """mg.cycle = 1: the W-cycle of Implementation/WCycle.m as the preconditioner
+    (SolveMG.m:31-37), against the oracle's restatement."""
+    _say(f"wcycle {precision} {nelx}x{nely}x{nelz} L={L}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    s = _setup(prob, x, L, precision, cycle=1)
+    b = np.random.default_rng(seed).standard_normal(prob.nfree)
+    assert _rel(s.vcycle(b), o.wcycle(b)) < (1e-8 if precision == "fp64" else 1e-5)
+    u_or, it_or, _ = o.solve(prob.f, cycle=1)
+    u, it, rr, rc = s.solve(prob.f)
+    assert rc == 0 and rr <= 1e-8
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    else:
+        assert it <= it_or + 3
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+
+
+def test_mixed_solve_escalates_to_fp64_preconditioner(oracle_mod, monkeypatch):
+    """A mixed-precision solve whose residual stops falling switches its
+    preconditioner to the FP64 hierarchy mid-solve and restarts the
+    directions. Forced here (a one-iteration window and an unreachable
+    reduction factor): the result still meets the stop rule against the
+    oracle's operator, and two forced solves are bitwise identical."""
+    _say("mixed escalation")
+    monkeypatch.setenv("TMG_STALL_WINDOW", "1")
+    monkeypatch.setenv("TMG_STALL_FACTOR", "1e-300")
+    prob = P.cantilever(16, 8, 8)
+    x = P.random_density(16, 8, 8, 12)
+    o = _oracle(oracle_mod, prob, x, 3)
+    u_or, it_or, _ = o.solve(prob.f)
+    s = _setup(prob, x, 3, "mixed")
+    u, it, rr, rc = s.solve(prob.f)
+    at, n = s.solve_stats()
+    _say(f"  switched at iteration {at}, {it} iterations")
+    assert n == 1 and 1 < at < it
+    assert rc == 0 and rr <= 1e-8
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+    u2, it2, _, _ = s.solve(prob.f)
+    assert s.solve_stats() == (at, 2)
+    assert it2 == it and np.array_equal(u2, u)
+    # without the forcing the same handle's solves do not switch
+    monkeypatch.delenv("TMG_STALL_WINDOW")
+    monkeypatch.delenv("TMG_STALL_FACTOR")
+    s2 = _setup(prob, x, 3, "mixed")
+    s2.solve(prob.f)
+    assert s2.solve_stats() == (0, 0)
+
+
 def test_config1_uniform_density(oracle_mod):
     """48x24x24, rho = 0.3, 4 levels, line load; also the closed form
     u(rho) = u(1) / (Emin + (1 - Emin) rho^3) for uniform densities."""

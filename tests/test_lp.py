@@ -125,3 +125,44 @@ def test_lp_large_instance_is_linear_time():
     assert ok
     _kkt(c, a, sL, sU, r, s, lam, False, tol=1e-9)
     assert dt < 5.0
+
+
+def test_lp_thread_count_does_not_change_the_step(monkeypatch):
+    n = 1_500_000
+    c, a, sL, sU, _ = _instance(n, 12)
+    r = a @ sL + 0.45 * (a @ sU - a @ sL)
+    out = []
+    for nt in ("1", "3", "16"):
+        monkeypatch.setenv("TMG_LP_THREADS", nt)
+        out.append(lp_volume(c, a, sL, sU, r))
+    for s, lam, ok in out[1:]:
+        assert ok and lam == out[0][1]
+        # every bound is identical; the one fractional element absorbs the
+        # rounding of the row sum (1.5M terms summed in per-thread order)
+        diff = np.flatnonzero(s != out[0][0])
+        assert diff.size <= 1
+        assert np.abs(s - out[0][0]).max() <= 1e-7
+
+
+def test_lp_large_tie_groups_across_threads(monkeypatch):
+    """8 distinct breakpoints shared by 200k elements each: the fractional
+    weight is spread over a tie group that spans several threads' chunks."""
+    monkeypatch.setenv("TMG_LP_THREADS", "8")
+    n = 1_600_000
+    rng = np.random.default_rng(13)
+    c = -np.repeat(np.linspace(1.0, 2.0, 8), n // 8)
+    a = np.ones(n)
+    sL, sU = -0.1 * rng.random(n), 0.1 * rng.random(n)
+    r = a @ sL + 0.37 * (a @ sU - a @ sL)
+    s, lam, ok = lp_volume(c, a, sL, sU, r)
+    assert ok
+    _kkt(c, a, sL, sU, r, s, lam, False, tol=1e-9)
+    frac = (s > sL + 1e-12) & (s < sU - 1e-12)
+    assert np.all(-c[frac] == lam)
+    # optimal value by the greedy closed form: fill the cheapest breakpoints first
+    order = np.argsort(c, kind="stable")
+    budget = r - a @ sL
+    take = np.minimum(np.maximum(budget - np.concatenate([[0.0], np.cumsum((sU - sL)[order])[:-1]]), 0),
+                      (sU - sL)[order])
+    best = c @ sL + c[order] @ take
+    assert abs(c @ s - best) <= 1e-9 * abs(best)

@@ -291,6 +291,63 @@ def test_two_level_cycle_closed_form(oracle_mod):
     assert b @ o.vcycle(b) > 0
 
 
+def test_wcycle_closed_forms(oracle_mod):
+    """W-cycle (Implementation/WCycle.m) pinned by what the recursion fixes:
+    with two levels it is the V-cycle; from an initial guess x0 one cycle is
+    the affine step x0 + B (b - A x0) of the zero-guess cycle B; three levels
+    apply the level-1 two-level cycle twice, B_W = 2 B1 - B1 A1 B1 on the
+    coarse residual; and the result stays symmetric positive definite."""
+    prob = P.cantilever(8, 4, 4)
+    x = P.random_density(8, 4, 4, 6)
+    rng = np.random.default_rng(5)
+    o2 = oracle_mod.Oracle(prob)
+    o2.assemble(x)
+    o2.setup_gmg(2, 0.6)
+    b = rng.standard_normal(o2.nfree(0))
+    assert np.array_equal(o2.wcycle(b), o2.vcycle(b))
+    o = oracle_mod.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(3, 0.6)
+    n0, n1 = o.nfree(0), o.nfree(1)
+    b1 = rng.standard_normal(n1)
+    x0 = rng.standard_normal(n1)
+    # level 1 sits above the coarsest level: its W-cycle from x0 is the
+    # two-level cycle's affine step
+    lhs = o.wcycle(b1, x0=x0, level=1)
+    rhs = x0 + o.vcycle(b1 - o.spmv(1, x0), level=1)
+    assert _rel(lhs, rhs) < 1e-11
+    # level 0: pre-smooth, then the coarse correction 2 B1 - B1 A1 B1
+    A = o.csr("A", 0).toarray()
+    Pm = o.csr("P", 0).toarray()
+    A1 = o.csr("A", 1).toarray()
+    B1 = np.column_stack([o.vcycle(e, level=1) for e in np.eye(n1)])
+    BW = 2 * B1 - B1 @ A1 @ B1
+    Minv = np.diag(0.6 / np.diag(A))
+    S = np.eye(n0) - Minv @ A
+    Ep = S @ (np.eye(n0) - Pm @ BW @ Pm.T @ A) @ S
+    B = (np.eye(n0) - Ep) @ np.linalg.inv(A)
+    b = rng.standard_normal(n0)
+    w = o.wcycle(b)
+    assert _rel(w, B @ b) < 1e-9
+    assert _rel(w, o.vcycle(b)) > 1e-6            # three levels: W differs from V
+    y = rng.standard_normal(n0)
+    assert abs(w @ y - b @ o.wcycle(y)) < 1e-10 * abs(b @ w)
+    assert b @ w > 0
+
+
+def test_wcycle_solve_matches_direct(oracle_mod):
+    prob = P.cantilever(16, 8, 8)
+    x = P.random_density(16, 8, 8, 7)
+    o = oracle_mod.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(3, 0.6)
+    u_v, it_v, _ = o.solve(prob.f)
+    u_w, it_w, rr = o.solve(prob.f, cycle=1)
+    assert rr <= 1e-8
+    assert _rel(u_w, u_v) < 1e-6
+    assert it_w <= it_v
+
+
 def test_smooth_is_damped_jacobi(oracle_mod):
     prob = P.cantilever(4, 2, 2)
     o = oracle_mod.Oracle(prob)

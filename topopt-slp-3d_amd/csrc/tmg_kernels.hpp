@@ -19,9 +19,11 @@ namespace tmg {
 constexpr int TPB = 256;
 
 // PCG state kept on the device so that an iteration needs no host round trip.
+// restart: the iteration whose direction is d = s (beta = 0): 1, or the
+// iteration at which a mixed-precision solve switched its preconditioner.
 struct PcgScalars {
     double rho, gamma, alpha, beta, dq, rr, nf, nr, tol;
-    int it, maxiter, done, status;
+    int it, maxiter, done, status, restart;
 };
 
 enum { EPI_PLAIN = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_DOT = 3 };
@@ -125,7 +127,7 @@ __device__ inline void ebe_source(const EbeArgs<T, TS> &a, int jx, int jy, int j
             for (int c = 0; c < 3; ++c) v[c] += a.mask[3 * node + c] ? T(0.125) * pe[c] : T(0);
         }
     } else {  // SRC_DIR (T = double)
-        const bool first = a.sc->it == 1;
+        const bool first = a.sc->it == a.sc->restart;
         const double beta = a.sc->beta;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -133,10 +135,47 @@ __device__ inline void ebe_source(const EbeArgs<T, TS> &a, int jx, int jy, int j
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8). xcd_block maps the dispatch index to a logical block so that each
+// XCD walks one contiguous range of blocks: neighbouring node planes are then
+// read through the same L2. A bijection on [0, nb) for any nb.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t q = nb >> 3, r = nb & 7, k = b & 7, m = b >> 3;
+    return k * q + (k < r ? k : r) + m;
+}
+
+// Cube elements (el = eh = ew): the 576 entries of Ke take only 10 distinct
+// magnitudes. With s_k = 1 when local nodes a and b share their coordinate in
+// direction k, sigma_k = 2 s_k - 1 and eps_a^k = 2 x_a^k - 1 (Q1 shape
+// functions integrated exactly; lambda, mu the Lame constants for E = 1):
+//   c == d:  h/36 [(l+2m) sig_c (1+s_k1)(1+s_k2) + m sig_k1 (1+s_c)(1+s_k2) + m sig_k2 (1+s_c)(1+s_k1)]
+//   c != d:  h/24 (1+s_e) eps_a^c eps_b^d (l + m sig_c sig_d),  e the third direction.
+// kc_class/kc_sign give each entry as sign * v[class]; after unrolling both
+// are compile-time constants, so every FMA reads its coefficient from one of
+// ten kernel-argument scalars that stay in SGPRs for the whole launch.
+template <typename T>
+struct KCube {
+    T v[10];
+};
+__host__ __device__ constexpr int kc_pos(int a, int k) { return k == 0 ? (a >> 1) & 1 : (k == 1 ? a & 1 : (a >> 2) & 1); }
+__host__ __device__ constexpr int kc_class(int a, int c, int b, int d) {
+    const int sc = kc_pos(a, c) == kc_pos(b, c), sd = kc_pos(a, d) == kc_pos(b, d);
+    if (c == d) {
+        const int k1 = c == 0 ? 1 : 0, k2 = c == 2 ? 1 : 2;
+        return sc * 3 + (kc_pos(a, k1) == kc_pos(b, k1)) + (kc_pos(a, k2) == kc_pos(b, k2));
+    }
+    const int e = 3 - c - d;
+    return 6 + (sc == sd ? 0 : 2) + (kc_pos(a, e) == kc_pos(b, e));
+}
+__host__ __device__ constexpr int kc_sign(int a, int c, int b, int d) {
+    return c == d ? 1 : (2 * kc_pos(a, c) - 1) * (2 * kc_pos(b, d) - 1);
+}
+
 // NSUB = 1: one modulus per element (E[e]); NSUB = 8: multiresolution with
 // 2x2x2 density sub-elements, E[8e + s] and eight sub-element integrands
 // K0[576 s + ...] (s = sy + 2 sx + 4 sz), element matrix sum_s E_s K_s.
-template <typename T, typename TS, int SRC, int EPI, bool TILED, int NSUB>
+// KC = true: cube elements, Ke from the ten scalars of kc_ (NSUB = 1 only).
+template <typename T, typename TS, int SRC, int EPI, bool TILED, int NSUB, bool KC = false>
 __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, int zchunk_, const T *__restrict__ x_,
                                              const T *__restrict__ E_, const T *__restrict__ K0_,
                                              const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
@@ -144,8 +183,17 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
                                              const T *__restrict__ ec_, int cnx_, int cny_, const TS *__restrict__ s_,
                                              const double *__restrict__ dold_, double *__restrict__ dnew_,
                                              const PcgScalars *__restrict__ sc_, T *__restrict__ out_,
-                                             double *__restrict__ partial_, const int *__restrict__ done_) {
+                                             double *__restrict__ partial_, const int *__restrict__ done_,
+                                             KCube<T> kc_, int xcd_) {
+    static_assert(!KC || NSUB == 1, "cube coefficients describe a single integrand");
     if (done_ && *done_) return;
+    // logical block (bx = strip of the plane, bz = z chunk)
+    int64_t bx = blockIdx.x, bz = blockIdx.y;
+    if (xcd_) {
+        const int64_t lb = xcd_block((int64_t)blockIdx.y * gridDim.x + blockIdx.x, (int64_t)gridDim.x * gridDim.y);
+        bz = lb / gridDim.x;
+        bx = lb - bz * gridDim.x;
+    }
     // the arguments arrive as non-aliasing kernel parameters and are bundled
     // here only for the helpers
     EbeArgs<T, TS> a;
@@ -155,12 +203,12 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
     const int nelx = a.nelx, nely = a.nely, nelz = a.nelz;
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t plane = (int64_t)nx * ny;
-    const int64_t p0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t p0 = bx * TPB + threadIdx.x;
     const bool act = p0 < plane;
     const int64_t p = act ? p0 : plane - 1;
     const int ix = (int)(p / ny);
     const int iy = (int)(p - (int64_t)ix * ny);
-    const int z0 = blockIdx.y * a.zchunk;
+    const int z0 = (int)bz * a.zchunk;
     const int z1 = min(z0 + a.zchunk, nz);
 
     // Neighbour indices are clamped into the grid, so every load is in
@@ -193,7 +241,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
 
     // ---- staging of node planes (TILED) ----
     __shared__ T tile[TILED ? 2 : 1][TILED ? 3 * EBE_TILE_W : 1];
-    const int64_t tile_lo = (int64_t)blockIdx.x * TPB - ny - 1;  // first plane index of the strip
+    const int64_t tile_lo = bx * TPB - ny - 1;                   // first plane index of the strip
     const int wn = TPB + 2 * ny + 2;                             // nodes per staged plane
     T pre[TILED ? EBE_TILE_NODES_PER_THREAD : 1][3];
     auto fetch_plane = [&](int zz) {                             // sources -> registers
@@ -320,6 +368,24 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
                 for (int half = 0; half < 2; ++half) {
                     if (half == 0 ? !do_bot : !do_top) continue;
                     const int ai = (1 - oy) + 2 * (1 - ox) + 4 * half;
+                    if constexpr (KC) {
+                        const T Ee = ew[ox][oy][0];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            T t = T(0);
+#pragma unroll
+                            for (int j = 0; j < 24; ++j) {
+                                const int bn = j / 3, d = j % 3;
+                                const int by = bn & 1, bx = (bn >> 1) & 1, bz = bn >> 2;
+                                const T kv = kc_.v[kc_class(ai, c, bn, d)];
+                                const T xv = xw[bz][ox + bx][oy + by][d];
+                                t = fma(kc_sign(ai, c, bn, d) > 0 ? kv : -kv, xv, t);
+                            }
+                            if (half == 0) ybot[c] += Ee * t;
+                            else ytop[c] += Ee * t;
+                        }
+                        continue;
+                    }
 #pragma unroll
                     for (int s = 0; s < NSUB; ++s) {
                         const T Ee = ew[ox][oy][s];
@@ -380,7 +446,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
         for (int c = 0; c < 3; ++c) ybot[c] = ytop[c];
     }
     if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT)
-        block_reduce_store(dacc, a.partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+        block_reduce_store(dacc, a.partial, bz * gridDim.x + bx);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,6 +462,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
 // ---------------------------------------------------------------------------
 constexpr int NCOEF = 126;
 __host__ __device__ inline int64_t sidx(int64_t i, int coef) { return (i >> 6) * (NCOEF * 64) + coef * 64 + (i & 63); }
+
 
 template <typename T>
 struct StArgs {
@@ -483,7 +550,7 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, cons
                                                  const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
                                                  const T *__restrict__ invD_, const T *__restrict__ ec_, int cnx_, int cny_,
                                                  T *__restrict__ out_, double *__restrict__ partial_,
-                                                 const int *__restrict__ done_) {
+                                                 const int *__restrict__ done_, int xcd_) {
     if (done_ && *done_) return;
     // the arguments arrive as non-aliasing kernel parameters and are bundled
     // here only for the helpers
@@ -493,7 +560,8 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, cons
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t nn = plane * nz;
-    const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t blk = xcd_ ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t i0 = blk * TPB + threadIdx.x;
     const bool act = i0 < nn;
     double dacc = 0.0;
     if (act) {
@@ -529,7 +597,7 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, cons
             }
         }
     }
-    if (EPI == EPI_DOT) block_reduce_store(dacc, a.partial, blockIdx.x);
+    if (EPI == EPI_DOT) block_reduce_store(dacc, a.partial, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,8 +701,13 @@ __global__ __launch_bounds__(TPB) void k_dir_update(int64_t n, const TS *__restr
     if (sc->done) return;
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
-    if (sc->it == 1) d[i] = (double)s[i];
+    if (sc->it == sc->restart) d[i] = (double)s[i];
     else d[i] = sc->beta * d[i] + (double)s[i];
+}
+
+// The next iteration starts a new conjugate direction sequence.
+__global__ void k_set_restart(PcgScalars *sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !sc->done) sc->restart = sc->it;
 }
 
 // x += alpha d; r -= alpha q; partial ||r||^2  (Implementation/SolveMG.m:47-50)
@@ -686,7 +759,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int np, const double *partial
             sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
         } else if (MODE == FIN_RHO) {     // rho = r' s; beta = rho / gamma2
             sc->rho = tot;
-            sc->beta = (sc->it == 1) ? 0.0 : tot / sc->gamma;
+            sc->beta = (sc->it == sc->restart) ? 0.0 : tot / sc->gamma;
         } else if (MODE == FIN_ALPHA) {   // alpha = rho / (d' q)
             sc->dq = tot;
             sc->alpha = sc->rho / tot;
@@ -969,6 +1042,61 @@ __global__ __launch_bounds__(TPB) void k_stencil_assemble(int nelx, int nely, in
                     const int aj = (jy - ey) + 2 * (jx - ex) + 4 * (jz - ez);
                     for (int c = 0; c < 3; ++c)
                         for (int d = 0; d < 3; ++d) acc[c][d] += Ahat[(int64_t)((3 * ai + c) * 24 + 3 * aj + d) * ne + e];
+                }
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                if (!(mask[3 * i + c] && mask[3 * j + d])) acc[c][d] = 0.0;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) {
+            const int64_t o = sidx(i, f * 9 + c * 3 + d);
+            S64[o] = acc[c][d];
+            if (S32) S32[o] = (T32)acc[c][d];
+        }
+}
+
+// Multiresolution fine level assembled into the symmetric block stencil: the
+// block between nodes i and j sums, over the elements containing both, the
+// element matrix sum_s E[8e+s] K_s (Implementation/GlobalStiffMatrix.m:19-62;
+// the same element matrices Implementation/WOK/KelemProd.m:58-83 applies).
+// Couplings to fixed DOFs are zeroed as on the Galerkin levels. One thread per
+// (node, forward block).
+template <typename T32>
+__global__ __launch_bounds__(TPB) void k_stencil_assemble_mr(int nelx, int nely, int nelz, const double *__restrict__ E,
+                                                             const double *__restrict__ K0,
+                                                             const uint8_t *__restrict__ mask, double *__restrict__ S64,
+                                                             T32 *__restrict__ S32) {
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const int64_t nn = (int64_t)nx * ny * nz;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= nn * 14) return;
+    const int f = (int)(t / nn);
+    const int k = 13 + f;
+    const int64_t i = t - (int64_t)f * nn;
+    const int iz = (int)(i / ((int64_t)nx * ny)), ix = (int)((i / ny) % nx), iy = (int)(i % ny);
+    const int jz = iz + k / 9 - 1, jx = ix + (k / 3) % 3 - 1, jy = iy + k % 3 - 1;
+    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const bool v = jz >= 0 && jz < nz && jx >= 0 && jx < nx && jy >= 0 && jy < ny;
+    if (v) {
+        const int64_t j = (int64_t)jz * nx * ny + (int64_t)jx * ny + jy;
+        for (int ez = max(iz, jz) - 1; ez <= min(iz, jz); ++ez)
+            for (int ex = max(ix, jx) - 1; ex <= min(ix, jx); ++ex)
+                for (int ey = max(iy, jy) - 1; ey <= min(iy, jy); ++ey) {
+                    if (ez < 0 || ez >= nelz || ex < 0 || ex >= nelx || ey < 0 || ey >= nely) continue;
+                    const int64_t e = (int64_t)ez * nelx * nely + (int64_t)ex * nely + ey;
+                    const int ai = (iy - ey) + 2 * (ix - ex) + 4 * (iz - ez);
+                    const int aj = (jy - ey) + 2 * (jx - ex) + 4 * (jz - ez);
+                    double es[8];
+#pragma unroll
+                    for (int sb = 0; sb < 8; ++sb) es[sb] = E[8 * e + sb];
+                    for (int c = 0; c < 3; ++c)
+                        for (int d = 0; d < 3; ++d) {
+                            const int o = (3 * ai + c) * 24 + 3 * aj + d;
+                            double a = 0.0;
+#pragma unroll
+                            for (int sb = 0; sb < 8; ++sb) a += es[sb] * K0[576 * sb + o];
+                            acc[c][d] += a;
+                        }
                 }
         for (int c = 0; c < 3; ++c)
             for (int d = 0; d < 3; ++d)

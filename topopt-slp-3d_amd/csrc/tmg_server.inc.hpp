@@ -45,7 +45,6 @@ struct SrvReq {
 size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfree) {
     size_t b = 0;
     auto add = [&](size_t n) { b += round256(std::max<size_t>(n, 1)); };
-    const size_t vs = prec == TMG_MIXED ? 4 : 8;
     int64_t ne1 = 1, ne2 = 1, ncoarse = 0;
     int ex = nelx, ey = nely, ez = nelz;
     int64_t ne0 = (int64_t)nelx * nely * nelz, ndof0 = 0, nn0x = nelx + 1, nn0y = nely + 1, nn0z = nelz + 1;
@@ -55,7 +54,7 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
         add(ndof);                        // mask
         add(8 * ndof);                    // d_free (upper bound: all free)
         add(8 * ndof);                    // d_cidx
-        add(vs * ndof); add(vs * ndof); add(vs * ndof);
+        add(8 * ndof); add(8 * ndof); add(8 * ndof);   // V-cycle work vectors (FP64-sized)
         add(8 * ndof);                    // inv64
         if (prec == TMG_MIXED) add(4 * ndof);
         if (l > 0) {
@@ -199,6 +198,8 @@ void srv_drive_loop(tmg_server *S) {
                     finished.push_back(r);
                     continue;
                 }
+                const int sck = solve_check(h);
+                if (sck < 0) { r->rc = sck; finished.push_back(r); continue; }
             }
             int rc = solve_launch(h, r->chunk);
             r->chunk = std::min(r->chunk * 2, 4);

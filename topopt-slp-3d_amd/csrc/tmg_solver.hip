@@ -39,6 +39,12 @@ struct DevLevel {
     int64_t *d_cidx = nullptr;                      // full dof -> compact index or -1
 };
 
+// true when the environment variable is set to a value starting with '1'
+bool getenv_flag(const char *name) {
+    const char *v = getenv(name);
+    return v && v[0] == '1';
+}
+
 }  // namespace
 
 struct tmg_solver {
@@ -73,8 +79,28 @@ struct tmg_solver {
     // the handle's lifetime), replayed GRAPH_ITERS iterations per launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // mixed precision: a solve whose residual stalls switches its
+    // preconditioner to the FP64 hierarchy (the handle keeps FP64 masters)
+    int pprec = TMG_FP64;             // preconditioner precision of the running solve
+    int vec_prec = -1;                // precision the V-cycle work vectors were last written in
+    hipGraph_t graph64 = nullptr;
+    hipGraphExec_t graph_exec64 = nullptr;
+    bool escalate = true;
+    int stall_it = 0;
+    double stall_nr = 0.0;
+    int escalations = 0;              // solves switched to the FP64 preconditioner
+    int escalated_at = 0;             // iteration of the last solve's switch (0: none)
+    int stall_window = 64;            // iterations between progress checks
+    double stall_factor = 0.5;        // required residual reduction per window
     bool use_graph = true;
     bool ebe_tiled = false;           // LDS-staged fine-level apply (ny <= EBE_TILE_MAXNY)
+    int xcd_remap = 1;                // XCD-contiguous block order in the stencil kernels
+    int xcd_ebe = 0;                  // the same for the fine-level apply (measured neutral or slower)
+    int zchunk64 = 8;                 // z planes per thread of the FP64 fine apply
+    bool cube = false;                // cube elements: Ke as ten scalars (KCube)
+    bool fine_stencil = false;        // fine level held as an assembled block stencil (multiresolution)
+    KCube<double> kc64{};
+    KCube<float> kc32{};
     bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
     hipStream_t st = nullptr;
@@ -138,6 +164,10 @@ void dfree(tmg_solver *h, T *&p) {
 template <typename T> const T *K0of(tmg_solver *h);
 template <> const double *K0of<double>(tmg_solver *h) { return h->K0_64; }
 template <> const float *K0of<float>(tmg_solver *h) { return h->K0_32; }
+template <typename T>
+const KCube<T> &KCof(tmg_solver *h);
+template <> const KCube<double> &KCof<double>(tmg_solver *h) { return h->kc64; }
+template <> const KCube<float> &KCof<float>(tmg_solver *h) { return h->kc32; }
 template <typename T> const T *Eof(tmg_solver *h);
 template <> const double *Eof<double>(tmg_solver *h) { return h->E64; }
 template <> const float *Eof<float>(tmg_solver *h) { return h->E32; }
@@ -149,11 +179,13 @@ template <> const double *invof<double>(const DevLevel &l) { return l.inv64; }
 template <> const float *invof<float>(const DevLevel &l) { return l.inv32; }
 
 // Number of blocks an apply of this level launches (partial-sum slots).
-int64_t apply_blocks(tmg_solver *h, int l) {
+// dbl: the FP64 launch (its z chunk differs from the FP32 one).
+int64_t apply_blocks(tmg_solver *h, int l, bool dbl) {
     const DevLevel &lv = h->lev[l];
-    if (l == 0) {
+    if (l == 0 && !h->fine_stencil) {
         int64_t pb = nblk((int64_t)lv.nx * lv.ny);
-        return pb * ((lv.nz + h->zchunk - 1) / h->zchunk);
+        const int zc = dbl ? h->zchunk64 : h->zchunk;
+        return pb * ((lv.nz + zc - 1) / zc);
     }
     return nblk(lv.nn);
 }
@@ -163,7 +195,8 @@ EbeArgs<T, TS> ebe_args(tmg_solver *h) {
     const DevLevel &l0 = h->lev[0];
     EbeArgs<T, TS> a;
     memset(&a, 0, sizeof(a));
-    a.nelx = l0.nelx; a.nely = l0.nely; a.nelz = l0.nelz; a.zchunk = h->zchunk;
+    a.nelx = l0.nelx; a.nely = l0.nely; a.nelz = l0.nelz;
+    a.zchunk = std::is_same<T, double>::value ? h->zchunk64 : h->zchunk;
     a.E = Eof<T>(h); a.K0 = K0of<T>(h); a.mask = l0.mask; a.invD = invof<T>(l0);
     a.cnx = h->L > 1 ? h->lev[1].nx : 1;
     a.cny = h->L > 1 ? h->lev[1].ny : 1;
@@ -173,16 +206,18 @@ EbeArgs<T, TS> ebe_args(tmg_solver *h) {
 template <typename T, typename TS, int SRC, int EPI>
 void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
     const DevLevel &lv = h->lev[0];
-    dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + h->zchunk - 1) / h->zchunk);
+    dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + a.zchunk - 1) / a.zchunk);
 #define EBE_ARGS a.nelx, a.nely, a.nelz, a.zchunk, a.x, a.E, a.K0, a.mask, a.b, a.invD, a.r, a.ec, a.cnx, \
-                 a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done
+                 a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KCof<T>(h), h->xcd_ebe
     if constexpr (SRC == SRC_X) {
         if (h->nsub == 8) {
             hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 8>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
             return;
         }
     }
-    if (lv.ny <= EBE_TILE_MAXNY && h->ebe_tiled)
+    if (h->cube && h->nsub == 1 && !h->ebe_tiled)
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1, true>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
+    else if (lv.ny <= EBE_TILE_MAXNY && h->ebe_tiled)
         hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, true, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
     else
         hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
@@ -190,7 +225,7 @@ void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
 }
 
 #define ST_ARGS(a) (a).nx, (a).ny, (a).nz, (a).x, (a).S, (a).mask, (a).b, (a).invD, (a).ec, (a).cnx, (a).cny, (a).out, \
-                   (a).partial, (a).done
+                   (a).partial, (a).done, h->xcd_remap
 
 template <typename T>
 StArgs<T> st_args(tmg_solver *h, int l) {
@@ -207,7 +242,7 @@ StArgs<T> st_args(tmg_solver *h, int l) {
 template <typename T, int EPI>
 void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial, const int *done) {
     const DevLevel &lv = h->lev[l];
-    if (l == 0) {
+    if (l == 0 && !h->fine_stencil) {
         EbeArgs<T, T> a = ebe_args<T, T>(h);
         a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
         launch_ebe<T, T, SRC_X, EPI>(h, a);
@@ -228,32 +263,56 @@ void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
 // One V(nu1,nu2) cycle from a zero initial guess at level l
 // (Implementation/VCycle.m:23-43). Returns the buffer holding the result.
 template <typename T>
-T *vcycle(tmg_solver *h, int l, const T *b, const int *done) {
+T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0 = nullptr);
+
+// Coarse correction of a cycle at level l: the direct solve below the
+// coarsest-but-one level, else one cycle (V) or two, the second from the
+// first's result (W, Implementation/WCycle.m:18-20).
+template <typename T>
+T *coarse_correction(tmg_solver *h, int l, const int *done) {
+    DevLevel &lc = h->lev[l + 1];
+    if (l + 1 == h->L - 1) {
+        coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);                      // eH = R \ (R' \ rH)
+        return (T *)lc.x;
+    }
+    T *e = vcycle<T>(h, l + 1, (const T *)lc.b, done);
+    if (h->mg.cycle == 1) e = vcycle<T>(h, l + 1, (const T *)lc.b, done, e);
+    return e;
+}
+
+// One cycle at level l (Implementation/VCycle.m:23-43; with mg.cycle = 1 the
+// recursion of Implementation/WCycle.m). x0: initial guess held in one of the
+// level's two work vectors, or nullptr for zero. Returns the buffer holding
+// the result (one of the level's two work vectors).
+template <typename T>
+T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
     DevLevel &lv = h->lev[l];
     DevLevel &lc = h->lev[l + 1];
     T *xa = (T *)lv.x, *xb = (T *)lv.t;
     const int nu1 = h->mg.smoothiter1, nu2 = h->mg.smoothiter2;
     // small stencil levels are launch-bound: fold the sweeps into the applies there
-    if (l > 0 && nu1 == 1 && nu2 == 1 && h->fused_coarse && lv.nn <= FUSED_LEVEL_MAXNODES) {
+    if (!x0 && l > 0 && nu1 == 1 && nu2 == 1 && h->fused_coarse && lv.nn <= FUSED_LEVEL_MAXNODES) {
         // stencil level, V(1,1): rh = b - A (M\b); then s = w + M\(b - A w), w = M\b + P eH
         StArgs<T> a = st_args<T>(h, l);
         a.b = b; a.out = xb; a.done = done;
         hipLaunchKernelGGL((k_stencil<T, SRC_JZ, EPI_RESID>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
         hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
                            lc.nz, xb, lc.mask, (T *)lc.b, done);
-        T *ec;
-        if (l + 1 == h->L - 1) {
-            coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);
-            ec = (T *)lc.x;
-        } else {
-            ec = vcycle<T>(h, l + 1, (const T *)lc.b, done);
-        }
+        T *ec = coarse_correction<T>(h, l, done);
         StArgs<T> c = st_args<T>(h, l);
         c.b = b; c.ec = ec; c.out = xa; c.done = done;
         hipLaunchKernelGGL((k_stencil<T, SRC_POST, EPI_JACOBI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(c));
         return xa;
     }
-    if (nu1 >= 1) {
+    if (x0) {
+        // sweeps from the given initial guess (Implementation/WCycle.m:3)
+        xa = x0;
+        xb = x0 == (T *)lv.x ? (T *)lv.t : (T *)lv.x;
+        for (int s = 0; s < nu1; ++s) {
+            apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
+            std::swap(xa, xb);
+        }
+    } else if (nu1 >= 1) {
         // first sweep from x0 = 0: x = M \ b (Implementation/Smooth.m:17 with x = 0)
         hipLaunchKernelGGL((k_jacobi_zero<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, b, invof<T>(lv), xa, done);
         for (int s = 1; s < nu1; ++s) {
@@ -266,13 +325,7 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done) {
     apply<T, EPI_RESID>(h, l, xa, b, xb, nullptr, done);                          // rh = b - A x
     hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st,    // rH = P' rh
                        lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, xb, lc.mask, (T *)lc.b, done);
-    T *ec;
-    if (l + 1 == h->L - 1) {
-        coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);                      // eH = R \ (R' \ rH)
-        ec = (T *)lc.x;
-    } else {
-        ec = vcycle<T>(h, l + 1, (const T *)lc.b, done);
-    }
+    T *ec = coarse_correction<T>(h, l, done);
     hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st,  // x += P eH
                        lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, (const T *)ec, lv.mask, xa, 1, done);
     for (int s = 0; s < nu2; ++s) {
@@ -298,13 +351,7 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
     launch_ebe<T, T, SRC_JZ, EPI_RESIDR>(h, a);
     hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(l1.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
                        l1.nz, (const T *)l0.t, l1.mask, (T *)l1.b, done);
-    T *ec;
-    if (h->L == 2) {
-        coarse_solve<T>(h, (const T *)l1.b, (T *)l1.x, done);
-        ec = (T *)l1.x;
-    } else {
-        ec = vcycle<T>(h, 1, (const T *)l1.b, done);
-    }
+    T *ec = coarse_correction<T>(h, 0, done);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.r = r; b.ec = ec; b.done = done; b.out = (T *)l0.x; b.partial = h->partial;
     launch_ebe<T, T, SRC_POST, EPI_POSTR>(h, b);
@@ -320,9 +367,9 @@ void precondition(tmg_solver *h, const double *r, const double **s64, const floa
     *s64 = nullptr;
     *s32 = nullptr;
     if (fused_ok(h)) {
-        if (h->prec == TMG_MIXED) *s32 = vcycle_fine_fused<float>(h, r, done);
+        if (h->pprec == TMG_MIXED) *s32 = vcycle_fine_fused<float>(h, r, done);
         else *s64 = vcycle_fine_fused<double>(h, r, done);
-    } else if (h->prec == TMG_MIXED) {
+    } else if (h->pprec == TMG_MIXED) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, (float *)l0.b, done);
         *s32 = vcycle<float>(h, 0, (const float *)l0.b, done);
     } else {
@@ -347,11 +394,12 @@ void pcg_iteration(tmg_solver *h, int parity) {
     const float *s32;
     const int nb = h->npartial;
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), nb);
-    const int napply = (int)apply_blocks(h, 0);
+    const int napply = (int)apply_blocks(h, 0, true);
     precondition(h, h->pr, &s64, &s32, done);
     if (fused_ok(h)) {
-        // rho = r' s (partials from the post-smoothing apply)
-        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
+        // rho = r' s (partials from the post-smoothing apply, preconditioner precision)
+        const int npost = (int)apply_blocks(h, 0, s32 == nullptr);
+        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, npost, h->partial, 0, nullptr, h->d_sc);
         double *dold = parity ? h->pd2 : h->pd, *dnew = parity ? h->pd : h->pd2;
         if (s32) direction_apply<float>(h, s32, dold, dnew, done);
         else direction_apply<double>(h, s64, dold, dnew, done);
@@ -381,6 +429,21 @@ int check_launch(tmg_solver *h) {
 
 // Start a solve on the handle's px (x0) and pf: r = f - A x0, norms, and the
 // PCG iteration graph (captured once per handle).
+// The V-cycle work vectors are shared by the FP32 and FP64 cycles of a mixed
+// handle. The coarse solve never writes fixed DOFs and relies on their being
+// zero, which holds only in the view they were zeroed or written in: switching
+// views re-zeroes every level's vectors (outside any graph capture).
+void use_cycle_precision(tmg_solver *h, int p) {
+    h->pprec = p;
+    if (h->vec_prec == p) return;
+    for (int l = 0; l < h->L; ++l) {
+        DevLevel &lv = h->lev[l];
+        for (void *v : {lv.b, lv.x, lv.t})
+            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0, (double *)v);
+    }
+    h->vec_prec = p;
+}
+
 int solve_begin(tmg_solver *h) {
     if (!h->have_op) {
         h->err = "no operator: call tmg_update_density first";
@@ -391,6 +454,11 @@ int solve_begin(tmg_solver *h) {
     memset(h->h_sc, 0, sizeof(PcgScalars));
     h->h_sc->tol = h->mg.tolMG;
     h->h_sc->maxiter = h->mg.maxiterMG;
+    h->h_sc->restart = 1;
+    use_cycle_precision(h, h->prec);
+    h->stall_it = 0;
+    h->stall_nr = 0.0;
+    h->escalated_at = 0;
     HIPX(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, h->st));
     // d = 0 so that the first direction update is exact
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->pd);
@@ -421,8 +489,9 @@ int solve_begin(tmg_solver *h) {
 // Launch nchunk x GRAPH_ITERS iterations, then an asynchronous copy of the
 // scalars to pinned memory and the completion event.
 int solve_launch(tmg_solver *h, int nchunk) {
-    if (h->graph_exec) {
-        for (int k = 0; k < nchunk; ++k) HIPX(hipGraphLaunch(h->graph_exec, h->st));
+    hipGraphExec_t ge = h->pprec == h->prec ? h->graph_exec : h->graph_exec64;
+    if (ge) {
+        for (int k = 0; k < nchunk; ++k) HIPX(hipGraphLaunch(ge, h->st));
     } else {
         for (int k = 0; k < nchunk * GRAPH_ITERS; ++k) pcg_iteration(h, k & 1);
     }
@@ -430,6 +499,33 @@ int solve_launch(tmg_solver *h, int nchunk) {
     HIPX(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->st));
     HIPX(hipEventRecord(h->ev, h->st));
     return 0;
+}
+
+// Called between chunks of a mixed-precision solve (h_sc current, not done):
+// when the recursive residual fell by less than 2x over the last 64 or more
+// iterations, the FP32 preconditioner has stalled (on designs with large
+// void regions its rounding hides the weak modes); the rest of the solve
+// uses the FP64 hierarchy and restarts the conjugate directions.
+int solve_check(tmg_solver *h) {
+    if (h->prec != TMG_MIXED || h->pprec != TMG_MIXED || !h->escalate) return 0;
+    const int it = h->h_sc->it;
+    const double nr = h->h_sc->nr;
+    if (h->stall_it == 0) { h->stall_it = it; h->stall_nr = nr; return 0; }
+    if (it - h->stall_it < h->stall_window) return 0;
+    if (!(nr > h->stall_factor * h->stall_nr)) { h->stall_it = it; h->stall_nr = nr; return 0; }
+    use_cycle_precision(h, TMG_FP64);
+    ++h->escalations;
+    h->escalated_at = it;
+    if (h->use_graph && !h->graph_exec64) {
+        HIPX(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < GRAPH_ITERS; ++k) pcg_iteration(h, k & 1);
+        hipGraph_t g = nullptr;
+        HIPX(hipStreamEndCapture(h->st, &g));
+        h->graph64 = g;
+        HIPX(hipGraphInstantiate(&h->graph_exec64, h->graph64, nullptr, nullptr, 0));
+    }
+    hipLaunchKernelGGL(k_set_restart, dim3(1), dim3(64), 0, h->st, h->d_sc);
+    return check_launch(h);
 }
 
 // 1 when the last launched chunk has completed (h_sc is then current), 0 if
@@ -460,6 +556,7 @@ int solve_resident(tmg_solver *h, int *it, double *relres) {
         TRY(solve_launch(h, chunk));
         HIPX(hipEventSynchronize(h->ev));
         if (h->h_sc->done) break;
+        TRY(solve_check(h));
         chunk = std::min(chunk * 2, 8);
     }
     return solve_result(h, it, relres);
@@ -487,22 +584,19 @@ int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
         HIPX(hipMemcpyAsync(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice, h->st));
     HIPX(hipMemcpyAsync(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice, h->st));
     HIPX(hipStreamSynchronize(h->st));  // host staging vectors go out of scope
-    const size_t vs = h->prec == TMG_MIXED ? sizeof(float) : sizeof(double);
-    TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * vs)));
-    TRY(dalloc(h, (char **)&lv.x, (int64_t)(lv.ndof * vs)));
-    TRY(dalloc(h, (char **)&lv.t, (int64_t)(lv.ndof * vs)));
-    // level vectors start at zero: fixed DOFs of a level vector are never
-    // written afterwards except with zero (the coarse solve skips them)
-    if (h->prec == TMG_MIXED) {
-        for (void *v : {lv.b, lv.x, lv.t})
-            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0f, (float *)v);
-    } else {
-        for (void *v : {lv.b, lv.x, lv.t})
-            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0, (double *)v);
-    }
+    // V-cycle work vectors are sized for FP64 in both modes: a mixed handle
+    // runs its FP64 cycle in them when a solve escalates (solve_check)
+    TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * sizeof(double))));
+    TRY(dalloc(h, (char **)&lv.x, (int64_t)(lv.ndof * sizeof(double))));
+    TRY(dalloc(h, (char **)&lv.t, (int64_t)(lv.ndof * sizeof(double))));
+    // level vectors start at zero (all-zero bytes: zero in either view): fixed
+    // DOFs of a level vector are never written afterwards except with zero
+    // (the coarse solve skips them)
+    for (void *v : {lv.b, lv.x, lv.t})
+        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, 0.0, (double *)v);
     TRY(dalloc(h, &lv.inv64, lv.ndof));
     if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.inv32, lv.ndof));
-    if (l > 0) {
+    if (l > 0 || h->fine_stencil) {
         const int64_t ns = ((lv.nn + 63) / 64) * 64 * NCOEF;  // wave-blocked symmetric stencil storage
         TRY(dalloc(h, &lv.S64, ns));
         if (h->prec == TMG_MIXED) TRY(dalloc(h, &lv.S32, ns));
@@ -529,6 +623,10 @@ void destroy_all(tmg_solver *h) {
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     h->graph_exec = nullptr;
+    if (h->graph_exec64) (void)hipGraphExecDestroy(h->graph_exec64);
+    if (h->graph64) (void)hipGraphDestroy(h->graph64);
+    h->graph_exec64 = nullptr;
+    h->graph64 = nullptr;
     h->graph = nullptr;
     if (h->rb && h->own_rb) rocblas_destroy_handle(h->rb);
     h->rb = nullptr;
@@ -594,6 +692,10 @@ int refresh(tmg_solver *h) {
                            h->E0, h->emin, h->penal, h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
     hipLaunchKernelGGL(k_diag_fine, dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->nsub, h->E64,
                        h->K0_64, l0.mask, om, l0.inv64, h->prec == TMG_MIXED ? l0.inv32 : nullptr);
+    if (h->fine_stencil)
+        hipLaunchKernelGGL((k_stencil_assemble_mr<float>), dim3(nblk(l0.nn * 14)), dim3(TPB), 0, h->st, l0.nelx,
+                           l0.nely, l0.nelz, h->E64, h->K0_64, l0.mask, l0.S64,
+                           h->prec == TMG_MIXED ? l0.S32 : nullptr);
     for (int l = 1; l < h->L; ++l) {
         DevLevel &lv = h->lev[l];
         double *dst = h->Ahat[(l - 1) & 1];
@@ -700,8 +802,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         g_setup_err = "invalid problem description";
         return TMG_ERR_ARG;
     }
-    if (mg->cycle != 0 || mg->smoother != 0 || mg->opchol != 0) {
-        g_setup_err = "only cycle=0 (V), smoother=0 (Jacobi), opchol=0 (coarse Cholesky) are supported";
+    if ((mg->cycle != 0 && mg->cycle != 1) || mg->smoother != 0 || mg->opchol != 0) {
+        g_setup_err = "only cycle=0 (V) or 1 (W), smoother=0 (Jacobi), opchol=0 (coarse Cholesky) are supported";
         return TMG_ERR_UNSUPPORTED;
     }
     if (mg->ngrids < 2 || mg->ngrids > MAXL || mg->smoothiter1 < 0 || mg->smoothiter2 < 0 || !(mg->omega > 0) ||
@@ -729,8 +831,17 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->L = mg->ngrids;
     h->mr = mr_n;
     h->nsub = mr_n * mr_n * mr_n;
+    h->pprec = precision;
+    // eight weighted integrands per element make the matrix-free apply 8x
+    // the work; the assembled stencil is read at memory speed instead
+    h->fine_stencil = h->nsub > 1 && !getenv_flag("TMG_MR_MATRIX_FREE");
     if (h->nsub > 1) h->fused = false;
+    if (const char *ev = getenv("TMG_XCD_REMAP")) h->xcd_remap = ev[0] == '1';
+    h->xcd_ebe = getenv_flag("TMG_XCD_EBE") ? 1 : 0;
     if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
+    h->escalate = !getenv_flag("TMG_NO_ESCALATE");
+    if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
+    if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
     if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
     if (const char *ev = getenv("TMG_FUSED_FINE")) h->fused = ev[0] == '1';
     if (const char *ev = getenv("TMG_UNFUSED_COARSE")) h->fused_coarse = ev[0] == '0';
@@ -806,6 +917,34 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     }
     std::vector<float> ke32(kop.size());
     for (size_t i = 0; i < kop.size(); ++i) ke32[i] = (float)kop[i];
+    // cube elements: Ke as ten scalars, taken from its own entries and
+    // accepted only if every entry is reproduced
+    h->cube = false;
+    if (h->nsub == 1 && h->el == h->eh && h->el == h->ew && !getenv_flag("TMG_EBE_GENERIC")) {
+        double v[10];
+        bool seen[10] = {false};
+        double kmax = 0.0;
+        for (int i = 0; i < 576; ++i) kmax = std::max(kmax, std::fabs(kop[i]));
+        for (int a = 0; a < 8; ++a)
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 8; ++b)
+                    for (int d = 0; d < 3; ++d) {
+                        const int k = kc_class(a, c, b, d);
+                        if (!seen[k]) { v[k] = kc_sign(a, c, b, d) * kop[(3 * a + c) * 24 + 3 * b + d]; seen[k] = true; }
+                    }
+        bool ok = true;
+        for (int k = 0; k < 10; ++k) ok = ok && seen[k];
+        for (int a = 0; ok && a < 8; ++a)
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 8; ++b)
+                    for (int d = 0; d < 3; ++d)
+                        ok = ok && std::fabs(kc_sign(a, c, b, d) * v[kc_class(a, c, b, d)] -
+                                             kop[(3 * a + c) * 24 + 3 * b + d]) <= 1e-13 * kmax;
+        if (ok) {
+            h->cube = true;
+            for (int k = 0; k < 10; ++k) { h->kc64.v[k] = v[k]; h->kc32.v[k] = (float)v[k]; }
+        }
+    }
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
     if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
@@ -820,10 +959,16 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     // z-chunk of the fine apply: enough blocks to fill the chip
     {
         const int64_t pb = nblk((int64_t)l0.nx * l0.ny);
+        // FP32 sweeps: about 4096 blocks; FP64 apply (half the occupancy):
+        // about 2048 blocks, i.e. longer z runs with fewer re-read halo planes
         int zc = (int)((l0.nz * pb + 4095) / 4096);
         h->zchunk = std::max(1, std::min(zc, 16));
+        int zc64 = (int)((l0.nz * pb + 2047) / 2048);
+        h->zchunk64 = std::max(1, std::min(zc64, 16));
+        if (const char *ev = getenv("TMG_ZCHUNK")) h->zchunk = std::max(1, std::min(atoi(ev), 64));
+        if (const char *ev = getenv("TMG_ZCHUNK64")) h->zchunk64 = std::max(1, std::min(atoi(ev), 64));
     }
-    h->npartial = (int)std::max<int64_t>(apply_blocks(h, 0), 4096);
+    h->npartial = (int)std::max<int64_t>(std::max(apply_blocks(h, 0, true), apply_blocks(h, 0, false)), 4096);
     if ((rc = dalloc(h, &h->partial, h->npartial)) || (rc = dalloc(h, &h->partial2, h->npartial)) ||
         (rc = dalloc(h, &h->d_sc, 1)))
         return fail(rc);
@@ -912,6 +1057,7 @@ int tmg_vcycle(tmg_solver *h, const double *b, double *x) {
     TRY(upload_free(h, 0, b, h->pr));
     const double *s64;
     const float *s32;
+    use_cycle_precision(h, h->prec);
     precondition(h, h->pr, &s64, &s32, nullptr);
     if (s32) hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, s32, h->pq, nullptr);
     else HIPX(hipMemcpyAsync(h->pq, s64, sizeof(double) * l0.ndof, hipMemcpyDeviceToDevice, h->st));
@@ -1020,8 +1166,11 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 6) return TMG_ERR_ARG;
     if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
     if (which == 2 && h->L < 3) return set_err(h, TMG_ERR_ARG, "level 2 is the coarsest level here");
+    if (which >= 4 && h->nsub > 1)
+        return set_err(h, TMG_ERR_UNSUPPORTED, "fused fine-level kernels have no multiresolution form");
     DevLevel &l0 = h->lev[0];
     const bool mixed = h->prec == TMG_MIXED;
+    use_cycle_precision(h, h->prec);
     // deterministic non-trivial inputs
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pd);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, 1.0, h->pr);
@@ -1085,6 +1234,13 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     (void)hipEventDestroy(e1);
     TRY(check_launch(h));
     *avg_ms = (double)ms / nrep;
+    return TMG_OK;
+}
+
+int tmg_solve_stats(const tmg_solver *h, int *escalated_at, int *escalations) {
+    if (!h) return TMG_ERR_ARG;
+    if (escalated_at) *escalated_at = h->escalated_at;
+    if (escalations) *escalations = h->escalations;
     return TMG_OK;
 }
 

@@ -4,7 +4,7 @@ gfx950 HIP kernels behind the C-ABI of include/tmg.h."""
 from . import problem
 from ._lib import TMG_FP64, TMG_MIXED, TmgError, lib
 from .gmg import (ApplyFilter, ElemNeighborhood, Filter, GlobalStiffMatrix, Grad, RefreshGMG, SetupGMG,
-                  Smooth, SolveMG, Solver, VCycle)
+                  Smooth, SolveMG, Solver, VCycle, WCycle)
 
 __all__ = ["problem", "lib", "TmgError", "TMG_FP64", "TMG_MIXED", "Solver", "Filter", "GlobalStiffMatrix",
-           "SetupGMG", "RefreshGMG", "SolveMG", "VCycle", "Smooth", "ElemNeighborhood", "ApplyFilter", "Grad"]
+           "SetupGMG", "RefreshGMG", "SolveMG", "VCycle", "WCycle", "Smooth", "ElemNeighborhood", "ApplyFilter", "Grad"]

@@ -51,8 +51,9 @@ class Solver:
     """Device-resident GMG-PCG hierarchy for one structured-grid problem."""
 
     def __init__(self, prob, ngrids, precision="fp64", omega=0.6, smoothiter1=1, smoothiter2=1,
-                 tol=1e-8, maxiter=5000, p=3.0, emin=1e-9, E0=None, mr=1):
+                 tol=1e-8, maxiter=5000, p=3.0, emin=1e-9, E0=None, mr=1, cycle=0):
         self.prob = prob
+        self.cycle = int(cycle)
         self.mr = int(mr)
         self.ndens = prob.nelem * self.mr ** 3   # densities / gradient entries (density mesh)
         self.precision = _PREC[precision]
@@ -61,7 +62,7 @@ class Solver:
         pr = _lib.Problem_t(prob.nelx, prob.nely, prob.nelz, prob.el, prob.eh, prob.ew,
                             float(prob.E if E0 is None else E0), prob.nu, float(p), float(emin),
                             fd.ctypes.data_as(C.POINTER(C.c_int64)), fd.size)
-        mg = _lib.MgParams_t(ngrids, 0, 0, omega, smoothiter1, smoothiter2, tol, maxiter, 0, 60000)
+        mg = _lib.MgParams_t(ngrids, self.cycle, 0, omega, smoothiter1, smoothiter2, tol, maxiter, 0, 60000)
         h = C.c_void_p()
         if self.mr == 1:
             rc = lib().tmg_setup(C.byref(pr), C.byref(mg), self.precision, C.byref(h))
@@ -169,6 +170,13 @@ class Solver:
         self._chk(lib().tmg_restrict(self.h, level, _dp(xf), _dp(xc)))
         return xc
 
+    def solve_stats(self):
+        """(iteration at which the last solve switched to the FP64 preconditioner or 0,
+        number of switches so far) -- mixed precision only."""
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().tmg_solve_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def time_kernel(self, which, nrep=20):
         """Average launch time (ms) of a solver kernel, HIP events on the handle's stream."""
         ms = C.c_double()
@@ -264,10 +272,10 @@ def SetupGMG(K, str, mg, elem, precision=None):
     s = Solver(str, int(_get(mg, "ngrids")), precision=prec, omega=float(_get(mg, "omega", 0.6)),
                smoothiter1=int(_get(mg, "smoothiter1", 1)), smoothiter2=int(_get(mg, "smoothiter2", 1)),
                tol=float(_get(mg, "tolMG", 1e-8)), maxiter=int(_get(mg, "maxiterMG", 5000)),
-               p=K.p, emin=K.emin)
-    if int(_get(mg, "cycle", 0)) != 0 or int(_get(mg, "smoother", 0)) != 0 or int(_get(mg, "opchol", 0)) != 0:
+               p=K.p, emin=K.emin, cycle=int(_get(mg, "cycle", 0)))
+    if int(_get(mg, "smoother", 0)) != 0 or int(_get(mg, "opchol", 0)) != 0:
         s.close()
-        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only the V-cycle, Jacobi smoother and coarse Cholesky are supported")
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only the Jacobi smoother and coarse Cholesky are supported")
     s.update_density(K.x)
     P = [_Prolongation(s, i) for i in range(1, s.ngrids)]
     return s, P, s, s, s, s, s
@@ -283,8 +291,10 @@ def SolveMG(x, f, Ac, P, M, L, U, R, perm, ngrids, cycle, smoother, smoothiter1,
             tolMG, maxiterMG, opchol, opcond, pcgp=None):
     """[x,it] = SolveMG(...) (Implementation/SolveMG.m:1-55). The hierarchy's
     smoothing counts, tolerance and iteration limit are those given here."""
-    if cycle != 0 or smoother != 0 or opchol != 0:
-        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only cycle=0, smoother=0, opchol=0 are supported")
+    if cycle not in (0, 1) or smoother != 0 or opchol != 0:
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only cycle=0/1, smoother=0, opchol=0 are supported")
+    if cycle != Ac.cycle:
+        raise TmgError(_lib.TMG_ERR_ARG, "cycle differs from the hierarchy's (SetupGMG's mg.cycle)")
     if ngrids != Ac.ngrids:
         raise TmgError(_lib.TMG_ERR_ARG, "ngrids does not match the hierarchy")
     if (smoothiter1, smoothiter2, float(tolMG), maxiterMG) != Ac.params:
@@ -297,6 +307,17 @@ def VCycle(A, b, x0, niter1, niter2, ngrids, nv, smoother, P, M, L, U, R, perm, 
     """x = VCycle(A,b,x0,...) (Implementation/VCycle.m:1-45) at the finest level, x0 = 0."""
     if nv != 1 or np.any(np.asarray(x0) != 0):
         raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "VCycle is exposed at nv = 1 with x0 = 0, as SolveMG calls it")
+    if A.cycle != 0:
+        raise TmgError(_lib.TMG_ERR_ARG, "the hierarchy was set up for W-cycles (use WCycle)")
+    return A.vcycle(b)
+
+
+def WCycle(A, b, x0, niter1, niter2, ngrids, nv, smoother, P, M, L, U, R, perm, opchol, opcond, pcgp=None):
+    """x = WCycle(A,b,x0,...) (Implementation/WCycle.m:1-27) at the finest level, x0 = 0."""
+    if nv != 1 or np.any(np.asarray(x0) != 0):
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "WCycle is exposed at nv = 1 with x0 = 0, as SolveMG calls it")
+    if A.cycle != 1:
+        raise TmgError(_lib.TMG_ERR_ARG, "the hierarchy was set up for V-cycles (use VCycle)")
     return A.vcycle(b)
 
 

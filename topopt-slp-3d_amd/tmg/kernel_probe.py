@@ -5,6 +5,9 @@ profiling with rocprofv3, e.g.
     rocprofv3 --pmc FETCH_SIZE -- python3 topopt-slp-3d_amd/tmg/kernel_probe.py --kernel 1 --precision mixed
     rocprofv3 --pmc FETCH_SIZE -- python3 topopt-slp-3d_amd/tmg/kernel_probe.py --copy-bytes 1073741824
 
+The copy mode runs an elementwise scale (read n floats, write n floats) whose
+HBM traffic is known, to calibrate the FETCH_SIZE / WRITE_SIZE counters.
+
 Kernel ids are those of tmg_time_kernel: 0 FP64 fine apply (q = K d),
 1 fine Jacobi sweep, 2 level-2 stencil sweep, 3 V-cycle, 4/5 fine fused
 residual / post-sweep, 6 fine fused direction update + apply. The workload is
@@ -32,11 +35,12 @@ def main():
     import torch
     torch.cuda.init()
     if a.copy_bytes:
+        # a plain streaming kernel of known traffic: read n floats, write n floats
         n = a.copy_bytes // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(a.reps):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.copy_bytes, "reps": a.reps}))
         return

@@ -164,3 +164,38 @@ def summarize(records):
             "mean_ms": float(lat.mean() * 1e3), "makespan_s": span, "requests_per_s": len(records) / span,
             "dof_iters_per_s": dofit / span,
             "all_converged": all(r["rc"] == 0 and r["relres"] <= 1e-8 for r in records)}
+
+
+def benchmark(reqs, budget_bytes=200e9, max_active=16, cap_sample=32, load=0.8, seed=4, precision="mixed",
+              log=None):
+    """Serving measurement: single-stream capacity from a serial run of the
+    first ``cap_sample`` requests (one slot, all queued at once), then the
+    whole stream replayed with Poisson arrivals at ``load`` x that capacity on
+    ``max_active`` slots. Returns the stream summary with the capacity, rate
+    and per-request latency percentiles."""
+    srv = Server(budget_bytes, max_active=1)
+    sample = reqs[:cap_sample]
+    t0 = srv.clock()
+    ids = [srv.submit(r["prob"], r["x"], r["ngrids"], precision=precision, t_arrival=t0) for r in sample]
+    done = []
+    for rid in ids:
+        srv.wait(rid)
+        done.append(srv.result(rid))
+        srv.release(rid)
+    span = max(d["t_done"] for d in done) - t0
+    srv.close()
+    cap = len(sample) / span
+    serial = [d["t_done"] - d["t_admit"] for d in done]
+    if log:
+        log(f"serving: serial capacity {cap:.2f} req/s over {len(sample)} requests "
+            f"(mean service {1e3 * float(np.mean(serial)):.1f} ms)")
+    rate = load * cap
+    srv = Server(budget_bytes, max_active=max_active)
+    recs = run_stream(srv, reqs, rate, seed=seed, precision=precision)
+    st = srv.stats()
+    srv.close()
+    out = summarize(recs)
+    out.update(single_stream_capacity_rps=cap, arrival_rate_rps=rate, load=load, max_active=max_active,
+               budget_bytes=budget_bytes, peak_bytes_reserved=st["peak_bytes_reserved"],
+               mean_serial_service_ms=1e3 * float(np.mean(serial)))
+    return out
