@@ -174,8 +174,9 @@ __host__ __device__ constexpr int kc_sign(int a, int c, int b, int d) {
 // NSUB = 1: one modulus per element (E[e]); NSUB = 8: multiresolution with
 // 2x2x2 density sub-elements, E[8e + s] and eight sub-element integrands
 // K0[576 s + ...] (s = sy + 2 sx + 4 sz), element matrix sum_s E_s K_s.
-// KC = true: cube elements, Ke from the ten scalars of kc_ (NSUB = 1 only).
-template <typename T, typename TS, int SRC, int EPI, bool TILED, int NSUB, bool KC = false>
+// KC = 1: cube elements, Ke from the ten scalars of kc_ (NSUB = 1 only);
+// KC = 2: the same with the next node plane loaded one step ahead.
+template <typename T, typename TS, int SRC, int EPI, bool TILED, int NSUB, int KC = 0>
 __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, int zchunk_, const T *__restrict__ x_,
                                              const T *__restrict__ E_, const T *__restrict__ K0_,
                                              const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
@@ -186,6 +187,8 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
                                              double *__restrict__ partial_, const int *__restrict__ done_,
                                              KCube<T> kc_, int xcd_) {
     static_assert(!KC || NSUB == 1, "cube coefficients describe a single integrand");
+    static_assert(KC != 2 || !TILED, "the prefetching form loads planes directly");
+    constexpr bool PF = KC == 2;
     if (done_ && *done_) return;
     // logical block (bx = strip of the plane, bz = z chunk)
     int64_t bx = blockIdx.x, bz = blockIdx.y;
@@ -284,6 +287,16 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
             for (int yy = 0; yy < 3; ++yy)
                 ebe_source<T, TS, SRC>(a, cx[xx], cy[yy], jz, zb + (int64_t)cx[xx] * ny + cy[yy], xw[1][xx][yy]);
     };
+    T xpre[PF ? 3 : 1][PF ? 3 : 1][3];                           // next plane, in flight during a step (PF)
+    auto load_pre = [&](int zz) {
+        const int jz = min(max(zz, 0), nz - 1);
+        const int64_t zb = (int64_t)jz * plane;
+#pragma unroll
+        for (int xx = 0; xx < (PF ? 3 : 0); ++xx)
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy)
+                ebe_source<T, TS, SRC>(a, cx[xx], cy[yy], jz, zb + (int64_t)cx[xx] * ny + cy[yy], xpre[xx][yy]);
+    };
     // moduli of layer ez; single resolution prefetches one layer ahead,
     // multiresolution (8x the values) loads at use
     T epre[2][2][1];
@@ -318,6 +331,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
         fetch_plane(z0);
     } else {
         load_plane(z0 - 1);
+        if (PF) load_pre(z0);
     }
     if (NSUB == 1) fetch_elems(z0 - 1, epre);
     double dacc = 0.0;
@@ -343,6 +357,14 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
             __syncthreads();
             window_from_tile(buf);
             fetch_plane(L + 2);       // in flight during this step's products
+        } else if (PF) {
+#pragma unroll
+            for (int xx = 0; xx < 3; ++xx)
+#pragma unroll
+                for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) xw[1][xx][yy][c] = xpre[PF ? xx : 0][PF ? yy : 0][c];
+            load_pre(L + 2);          // in flight during this step's products
         } else {
             load_plane(L + 1);
         }
@@ -662,12 +684,12 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(int fnx, int fny, int fnz, 
 // ---------------------------------------------------------------------------
 // Vector kernels.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_jacobi_zero(int64_t n, const T *__restrict__ b, const T *__restrict__ invD,
+template <typename T, typename TB = T>
+__global__ __launch_bounds__(TPB) void k_jacobi_zero(int64_t n, const TB *__restrict__ b, const T *__restrict__ invD,
                                                      T *__restrict__ x, const int *done) {
     if (done && *done) return;
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i < n) x[i] = invD[i] * b[i];
+    if (i < n) x[i] = invD[i] * (T)b[i];
 }
 
 template <typename TD, typename TS>

@@ -98,11 +98,13 @@ struct tmg_solver {
     int xcd_ebe = 0;                  // the same for the fine-level apply (measured neutral or slower)
     int zchunk64 = 8;                 // z planes per thread of the FP64 fine apply
     bool cube = false;                // cube elements: Ke as ten scalars (KCube)
+    bool ebe_prefetch = true;         // cube fine apply loading the next plane one step ahead
     bool fine_stencil = false;        // fine level held as an assembled block stencil (multiresolution)
     KCube<double> kc64{};
     KCube<float> kc32{};
     bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
+    bool fine_r = true;               // fine-level cycle reading the FP64 residual, r's in its post-sweep
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
     // externally owned resources (a server slot): device memory comes from a
@@ -215,8 +217,10 @@ void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
             return;
         }
     }
-    if (h->cube && h->nsub == 1 && !h->ebe_tiled)
-        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1, true>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
+    if (h->cube && h->nsub == 1 && !h->ebe_tiled && h->ebe_prefetch)
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1, 2>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
+    else if (h->cube && h->nsub == 1 && !h->ebe_tiled)
+        hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 1, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
     else if (lv.ny <= EBE_TILE_MAXNY && h->ebe_tiled)
         hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, true, 1>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
     else
@@ -360,12 +364,55 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
 
 bool fused_ok(const tmg_solver *h) { return h->fused && h->nsub == 1 && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1; }
 
+// Fine-level V(1,1) cycle on the FP64 PCG residual r with the conversion and
+// the PCG's r's folded into kernels the cycle runs anyway
+// (Implementation/VCycle.m:23-43 with x0 = 0):
+//   x  = M \ r                     (first sweep from zero, reading r)
+//   rh = r - A x                    (residual epilogue reading r)
+//   eH = coarse correction (P' rh); x += P eH
+//   s  = x + M \ (r - A x)          (post-sweep, partial[] += r . s)
+// Returns s; the partial sums come from the post-sweep's blocks.
+template <typename T>
+T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done) {
+    DevLevel &l0 = h->lev[0];
+    DevLevel &l1 = h->lev[1];
+    T *xa = (T *)l0.x, *xb = (T *)l0.t;
+    hipLaunchKernelGGL((k_jacobi_zero<T, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, invof<T>(l0),
+                       xa, done);
+    EbeArgs<T, T> a = ebe_args<T, T>(h);
+    a.x = xa; a.r = r; a.out = xb; a.done = done;
+    launch_ebe<T, T, SRC_X, EPI_RESIDR>(h, a);
+    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(l1.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+                       l1.nz, (const T *)xb, l1.mask, (T *)l1.b, done);
+    T *ec = coarse_correction<T>(h, 0, done);
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+                       l1.nz, (const T *)ec, l0.mask, xa, 1, done);
+    EbeArgs<T, T> b = ebe_args<T, T>(h);
+    b.x = xa; b.r = r; b.out = xb; b.partial = h->partial; b.done = done;
+    launch_ebe<T, T, SRC_X, EPI_POSTR>(h, b);
+    return xb;
+}
+
+// the fine level is matrix-free and the cycle is V(1,1) or W(1,1)
+bool fine_r_ok(const tmg_solver *h) {
+    return h->fine_r && !h->fine_stencil && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1;
+}
+
 // Preconditioner application s = B r for the PCG; r is the FP64 residual.
-// Returns s as a typed pointer through one of the two out-params.
-void precondition(tmg_solver *h, const double *r, const double **s64, const float **s32, const int *done) {
+// Returns s as a typed pointer through one of the two out-params; *nrho > 0
+// when partial[0..*nrho) already hold the blocks' sums of r . s.
+void precondition(tmg_solver *h, const double *r, const double **s64, const float **s32, const int *done,
+                  int *nrho = nullptr) {
     DevLevel &l0 = h->lev[0];
     *s64 = nullptr;
     *s32 = nullptr;
+    if (nrho) *nrho = 0;
+    if (!fused_ok(h) && fine_r_ok(h)) {
+        if (h->pprec == TMG_MIXED) *s32 = vcycle_fine_r<float>(h, r, done);
+        else *s64 = vcycle_fine_r<double>(h, r, done);
+        if (nrho) *nrho = (int)apply_blocks(h, 0, h->pprec != TMG_MIXED);
+        return;
+    }
     if (fused_ok(h)) {
         if (h->pprec == TMG_MIXED) *s32 = vcycle_fine_fused<float>(h, r, done);
         else *s64 = vcycle_fine_fused<double>(h, r, done);
@@ -395,7 +442,8 @@ void pcg_iteration(tmg_solver *h, int parity) {
     const int nb = h->npartial;
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), nb);
     const int napply = (int)apply_blocks(h, 0, true);
-    precondition(h, h->pr, &s64, &s32, done);
+    int nrho = 0;
+    precondition(h, h->pr, &s64, &s32, done, &nrho);
     if (fused_ok(h)) {
         // rho = r' s (partials from the post-smoothing apply, preconditioner precision)
         const int npost = (int)apply_blocks(h, 0, s32 == nullptr);
@@ -406,9 +454,14 @@ void pcg_iteration(tmg_solver *h, int parity) {
         hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
         hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, dnew, h->pq, h->partial, h->d_sc);
     } else {
-        if (s32) hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
-        else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
-        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+        if (nrho > 0) {
+            // rho = r' s from the post-sweep's partial sums
+            hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, nrho, h->partial, 0, nullptr, h->d_sc);
+        } else {
+            if (s32) hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
+            else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
+            hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+        }
         if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
         else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
         apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);          // q = A d, d'q
@@ -840,6 +893,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->xcd_ebe = getenv_flag("TMG_XCD_EBE") ? 1 : 0;
     if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
     h->escalate = !getenv_flag("TMG_NO_ESCALATE");
+    if (const char *ev = getenv("TMG_FINE_R")) h->fine_r = ev[0] != '0';
+    if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
     if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
