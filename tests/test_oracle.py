@@ -468,3 +468,33 @@ def test_filter_fixed_elements_decoupled(oracle_mod):
         r = W.getrow(e)
         assert r.nnz == 1 and r[0, e] == 1.0
         assert W.getcol(e).nnz == 1
+
+
+def test_problem_file_round_trip(tmp_path):
+    """problem.from_mat reads a str struct (as the reference's problem files
+    hold it) and rebuilds freedofs/f from its rows; a file whose stored
+    vectors disagree with its rows is rejected."""
+    import scipy.io as sio
+    prob = P.from_rows(12, 4, 4, [[0, 0, 0, 4, 0, 4, 1, 1, 1]], [[12, 12, 0, 0, 0, 4, 0, -1, 0]])
+    st = {"nelx": 12, "nely": 4, "nelz": 4, "l": 12.0, "h": 4.0, "w": 4.0, "E": 1.0, "nu": 0.3,
+          "supp": np.array([[0, 0, 0, 4, 0, 4, 1, 1, 1]], float),
+          "loads": np.array([[12, 12, 0, 0, 0, 4, 0, -1, 0]], float),
+          "freedofs": prob.freedofs.astype(float), "f": prob.f}
+    path = tmp_path / "p.mat"
+    sio.savemat(path, {"str": st})
+    q = P.from_mat(path)
+    assert np.array_equal(q.freedofs, prob.freedofs) and np.array_equal(q.f, prob.f)
+    st["f"] = prob.f * 2
+    sio.savemat(path, {"str": st})
+    with pytest.raises(ValueError):
+        P.from_mat(path)
+    assert P.from_mat(path, check=False).nfree == prob.nfree
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/Implementation/Tests"), reason="reference problem files absent")
+def test_reference_problem_files_read_back():
+    """Every reference problem file with point/line loads rebuilds to its
+    stored freedofs and f (the setup fixtures' source)."""
+    files = [fx["file"] for fx in json.load(open(os.path.join(HERE, "golden", "setup_fixtures.json")))]
+    for name in files[:8]:
+        P.from_mat(os.path.join("/root/reference/Implementation/Tests", name))

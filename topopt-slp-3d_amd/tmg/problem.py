@@ -155,6 +155,32 @@ def from_rows(nelx, nely, nelz, supp, loads, l=None, h=None, w=None, E=1.0, nu=0
                    supp, loads)
 
 
+def from_mat(path, check=True) -> Problem:
+    """Read a reference problem file (the ``str`` struct of a ``.mat`` as the
+    reference stores them, e.g. ``Implementation/Tests/cb48x16x16.mat``) with
+    ``scipy.io.loadmat``, which executes nothing, and rebuild it through
+    :func:`from_rows` from its support and load rows. With ``check`` the
+    rebuilt ``freedofs`` and ``f`` must equal the stored ones."""
+    import scipy.io as sio
+    d = sio.loadmat(path, squeeze_me=True, struct_as_record=False)
+    if "str" not in d:
+        raise ValueError(f"{path}: no 'str' struct")
+    st = d["str"]
+    names = set(st._fieldnames)
+    for k in ("nelx", "nely", "nelz", "supp", "loads"):
+        if k not in names:
+            raise ValueError(f"{path}: str has no field {k!r}")
+    get = lambda k, dflt=None: getattr(st, k) if k in names else dflt
+    prob = from_rows(int(st.nelx), int(st.nely), int(st.nelz), get("supp"), get("loads"),
+                     l=get("l"), h=get("h"), w=get("w"), E=float(get("E", 1.0)), nu=float(get("nu", 0.3)))
+    if check and "freedofs" in names and "f" in names:
+        fd = np.asarray(st.freedofs, dtype=np.int64).ravel()
+        f = np.asarray(st.f, dtype=np.float64).ravel()
+        if not (np.array_equal(fd, prob.freedofs) and np.allclose(f, prob.f, rtol=0, atol=1e-12)):
+            raise ValueError(f"{path}: stored freedofs/f differ from the ones rebuilt from supp/loads")
+    return prob
+
+
 def cantilever(nelx, nely, nelz, load="line", tip=None) -> Problem:
     """Cantilever of the benchmark workloads: face x = 0 clamped; either a unit
     downward line load along z at (x = nelx, y = 0), or a unit point load
