@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--solve-only", default=None, choices=["fp64", "mixed"],
+                    help="run only this solve (profiling aid; prints no JSON line)")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--workload", default="solve", choices=["solve", "slp"],
                     help="solve: MG-PCG solves + serving stream (default); slp: the multiresolution SLP loop")
@@ -163,7 +165,7 @@ def main():
 
     results = {}
     solvers = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ((args.solve_only,) if args.solve_only else ("fp64", "mixed")):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s = tmg.Solver(prob, L, precision=prec)
@@ -185,6 +187,9 @@ def main():
             torch.cuda.synchronize()
             times.append(time.perf_counter() - ts)
             its.append(it)
+        if args.solve_only:
+            log(f"{prec}: it={its[-1]} t_solve={1e3 * sum(times) / len(times):.3f} ms")
+            return
         u = xd.cpu().numpy()[prob.freedofs - 1]
         # true residual with the FP64 operator of the FP64 hierarchy
         Ku = solvers["fp64"].apply(1, u)

@@ -115,7 +115,9 @@ def test_wcycle_matches_oracle(oracle_mod, precision, nelx, nely, nelz, L, seed)
     o = _oracle(oracle_mod, prob, x, L)
     s = _setup(prob, x, L, precision, cycle=1)
     b = np.random.default_rng(seed).standard_normal(prob.nfree)
-    assert _rel(s.vcycle(b), o.wcycle(b)) < (1e-8 if precision == "fp64" else 1e-5)
+    # FP64: explicit coarse inverse vs triangular solves; mixed: the FP32
+    # cycle's rounding (eps32 = 6e-8) amplified through the two coarse visits
+    assert _rel(s.vcycle(b), o.wcycle(b)) < (1e-8 if precision == "fp64" else 5e-5)
     u_or, it_or, _ = o.solve(prob.f, cycle=1)
     u, it, rr, rc = s.solve(prob.f)
     assert rc == 0 and rr <= 1e-8

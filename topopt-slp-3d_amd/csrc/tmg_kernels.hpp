@@ -21,10 +21,39 @@ constexpr int TPB = 256;
 // PCG state kept on the device so that an iteration needs no host round trip.
 // restart: the iteration whose direction is d = s (beta = 0): 1, or the
 // iteration at which a mixed-precision solve switched its preconditioner.
+// ticket: blocks of a reduction kernel that finished (the last one completes
+// the reduction and resets it to 0).
 struct PcgScalars {
     double rho, gamma, alpha, beta, dq, rr, nf, nr, tol;
     int it, maxiter, done, status, restart;
+    unsigned int ticket;
 };
+
+enum { FIN_NONE = -1, FIN_INIT = 0, FIN_RHO = 1, FIN_ALPHA = 2, FIN_RR = 3 };
+
+// The scalar PCG recurrences (Implementation/SolveMG.m:38-50) once a
+// reduction total is known; tot2 only for FIN_INIT (||f||^2).
+__device__ inline void fin_update(int mode, double tot, double tot2, PcgScalars *sc) {
+    if (mode == FIN_INIT) {               // r = f - A x0; nr = ||r||/||f||; it = 1
+        sc->nf = sqrt(tot2);
+        sc->rr = tot;
+        sc->nr = sqrt(tot) / sc->nf;
+        sc->it = 1;
+        sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
+    } else if (mode == FIN_RHO) {         // rho = r' s; beta = rho / gamma2
+        sc->rho = tot;
+        sc->beta = (sc->it == sc->restart) ? 0.0 : tot / sc->gamma;
+    } else if (mode == FIN_ALPHA) {       // alpha = rho / (d' q)
+        sc->dq = tot;
+        sc->alpha = sc->rho / tot;
+    } else {                              // gamma2 = rho; nr = ||r||/||f||; it++
+        sc->rr = tot;
+        sc->gamma = sc->rho;
+        sc->nr = sqrt(tot) / sc->nf;
+        sc->it = sc->it + 1;
+        sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
+    }
+}
 
 enum { EPI_PLAIN = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_DOT = 3 };
 
@@ -43,6 +72,39 @@ __device__ inline void block_reduce_store(double v, double *partial, int64_t slo
 #pragma unroll
         for (int w = 0; w < TPB / 64; ++w) s += red[w];
         partial[slot] = s;
+    }
+}
+
+// block_reduce_store, then the last of the kernel's nb blocks to finish sums
+// partial[0..nb) in a fixed order and applies the PCG recurrence `mode`
+// (the work of k_finalize, without its launch). Ticket protocol: each block's
+// thread 0 publishes its partial (release fence) before taking a ticket; the
+// block holding the last ticket fences (acquire) and reads the partials
+// through agent-scope loads, which bypass the non-coherent per-CU cache.
+__device__ inline void finish_reduce(int mode, double v, double *partial, int64_t slot, int64_t nb, PcgScalars *sc) {
+    block_reduce_store(v, partial, slot);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(&sc->ticket, 1u);
+        last = t == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < nb; i += TPB) s += __hip_atomic_load(partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ double red[TPB / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) tot += red[w];
+        fin_update(mode, tot, 0.0, sc);
+        sc->ticket = 0u;
     }
 }
 
@@ -82,7 +144,8 @@ struct EbeArgs {
     const TS *s;           // preconditioned residual (SRC_DIR)
     const double *dold;    // previous direction (SRC_DIR)
     double *dnew;          // new direction (EPI_DIRDOT)
-    const PcgScalars *sc;  // beta, it (SRC_DIR)
+    PcgScalars *sc;        // beta, it (SRC_DIR); reduction finish (fin >= 0)
+    int fin;               // FIN_* applied by the last block of a DOT-type epilogue
     T *out;
     double *partial;       // per-block partial sums (DOT, POSTR, DIRDOT)
     const int *done;       // skip once a PCG has converged
@@ -183,9 +246,9 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
                                              const T *__restrict__ invD_, const double *__restrict__ r_,
                                              const T *__restrict__ ec_, int cnx_, int cny_, const TS *__restrict__ s_,
                                              const double *__restrict__ dold_, double *__restrict__ dnew_,
-                                             const PcgScalars *__restrict__ sc_, T *__restrict__ out_,
+                                             PcgScalars *__restrict__ sc_, T *__restrict__ out_,
                                              double *__restrict__ partial_, const int *__restrict__ done_,
-                                             KCube<T> kc_, int xcd_) {
+                                             KCube<T> kc_, int xcd_, int fin_) {
     static_assert(!KC || NSUB == 1, "cube coefficients describe a single integrand");
     static_assert(KC != 2 || !TILED, "the prefetching form loads planes directly");
     constexpr bool PF = KC == 2;
@@ -202,7 +265,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
     EbeArgs<T, TS> a;
     a.nelx = nelx_; a.nely = nely_; a.nelz = nelz_; a.zchunk = zchunk_; a.x = x_; a.E = E_; a.K0 = K0_;
     a.mask = mask_; a.b = b_; a.invD = invD_; a.r = r_; a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.s = s_;
-    a.dold = dold_; a.dnew = dnew_; a.sc = sc_; a.out = out_; a.partial = partial_; a.done = done_;
+    a.dold = dold_; a.dnew = dnew_; a.sc = sc_; a.out = out_; a.partial = partial_; a.done = done_; a.fin = fin_;
     const int nelx = a.nelx, nely = a.nely, nelz = a.nelz;
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t plane = (int64_t)nx * ny;
@@ -467,8 +530,12 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
 #pragma unroll
         for (int c = 0; c < 3; ++c) ybot[c] = ytop[c];
     }
-    if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT)
-        block_reduce_store(dacc, a.partial, bz * gridDim.x + bx);
+    if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT) {
+        if (a.fin >= 0)
+            finish_reduce(a.fin, dacc, a.partial, bz * gridDim.x + bx, (int64_t)gridDim.x * gridDim.y, a.sc);
+        else
+            block_reduce_store(dacc, a.partial, bz * gridDim.x + bx);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -627,52 +694,87 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, cons
 // 1/8, Implementation/Prolongation.m:15-37) restricted to free DOFs, and
 // restriction P'.
 // ---------------------------------------------------------------------------
+// Launch shape of both transfers: blockIdx.y = z plane of the output level,
+// blockIdx.x * TPB + threadIdx.x = in-plane index x * ny + y (32-bit).
+// Index arithmetic is per thread, once; boundary neighbours are clamped and
+// given weight zero, so there are no branches.
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_restrict(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
                                                   const T *__restrict__ rf, const uint8_t *__restrict__ cmask,
                                                   T *__restrict__ bc, const int *done) {
     if (done && *done) return;
-    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    const int64_t cn = (int64_t)cnx * cny * cnz;
-    if (I >= cn) return;
-    const int Iz = (int)(I / ((int64_t)cnx * cny)), Ix = (int)((I / cny) % cnx), Iy = (int)(I % cny);
+    const int p = (int)blockIdx.x * TPB + (int)threadIdx.x;
+    if (p >= cnx * cny) return;
+    const int Iz = (int)blockIdx.y, Ix = p / cny, Iy = p - Ix * cny;
     const int64_t fplane = (int64_t)fnx * fny;
+    // fine neighbours 2I-1, 2I, 2I+1 per direction, weights 1/2, 1, 1/2
+    // (Implementation/Prolongation.m:15,34: w = 2^-(squared offset))
+    const T wlz = Iz > 0 ? T(0.5) : T(0), whz = Iz < cnz - 1 ? T(0.5) : T(0);
+    const T wlx = Ix > 0 ? T(0.5) : T(0), whx = Ix < cnx - 1 ? T(0.5) : T(0);
+    const T wly = Iy > 0 ? T(0.5) : T(0), why = Iy < cny - 1 ? T(0.5) : T(0);
+    const int fz = 2 * Iz, fx = 2 * Ix, fy = 2 * Iy;
+    const int64_t oz[3] = {(int64_t)max(fz - 1, 0) * fplane, (int64_t)fz * fplane, (int64_t)min(fz + 1, fnz - 1) * fplane};
+    const int64_t ox[3] = {(int64_t)max(fx - 1, 0) * fny, (int64_t)fx * fny, (int64_t)min(fx + 1, fnx - 1) * fny};
+    const int oy[3] = {max(fy - 1, 0), fy, min(fy + 1, fny - 1)};
+    const T wz[3] = {wlz, T(1), whz}, wx[3] = {wlx, T(1), whx}, wy[3] = {wly, T(1), why};
     T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const int dz = k / 9 - 1, dx = (k / 3) % 3 - 1, dy = k % 3 - 1;
-        const int fz = 2 * Iz + dz, fx = 2 * Ix + dx, fy = 2 * Iy + dy;
-        if (fz < 0 || fz >= fnz || fx < 0 || fx >= fnx || fy < 0 || fy >= fny) continue;
-        const T w = T(1.0 / (double)(1 << (dz * dz + dx * dx + dy * dy)));
-        const int64_t j = (int64_t)fz * fplane + (int64_t)fx * fny + fy;
+    for (int a = 0; a < 3; ++a) {
+        T sx[3] = {T(0), T(0), T(0)};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += w * rf[3 * j + c];
+        for (int b = 0; b < 3; ++b) {
+            T sy[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int64_t j = oz[a] + ox[b] + oy[d];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sy[c] += wy[d] * rf[3 * j + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sx[c] += wx[b] * sy[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += wz[a] * sx[c];
     }
+    const int64_t I = (int64_t)Iz * cnx * cny + p;
 #pragma unroll
     for (int c = 0; c < 3; ++c) bc[3 * I + c] = cmask[3 * I + c] ? acc[c] : T(0);
 }
 
+// x_f (+)= P e_c on free fine DOFs: each fine node takes its co-located or
+// two/four/eight neighbouring coarse nodes with weights 1, 1/2, 1/4, 1/8.
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_prolong_add(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
                                                      const T *__restrict__ ec, const uint8_t *__restrict__ fmask,
                                                      T *__restrict__ xf, int add, const int *done) {
     if (done && *done) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    const int64_t fn = (int64_t)fnx * fny * fnz;
-    if (i >= fn) return;
-    const int iz = (int)(i / ((int64_t)fnx * fny)), ix = (int)((i / fny) % fnx), iy = (int)(i % fny);
+    const int p = (int)blockIdx.x * TPB + (int)threadIdx.x;
+    if (p >= fnx * fny) return;
+    const int iz = (int)blockIdx.y, ix = p / fny, iy = p - ix * fny;
     const int64_t cplane = (int64_t)cnx * cny;
-    T acc[3] = {T(0), T(0), T(0)};
+    // even index: the co-located coarse node (weight 1); odd: the two
+    // neighbours (1/2 each)
     const int zl = iz >> 1, zh = (iz + 1) >> 1, xl = ix >> 1, xh = (ix + 1) >> 1, yl = iy >> 1, yh = (iy + 1) >> 1;
-    const T wz = (zl == zh) ? T(1) : T(0.5), wx = (xl == xh) ? T(1) : T(0.5), wy = (yl == yh) ? T(1) : T(0.5);
-    for (int Z = zl; Z <= zh; ++Z)
-        for (int X = xl; X <= xh; ++X)
-            for (int Y = yl; Y <= yh; ++Y) {
-                const int64_t J = (int64_t)Z * cplane + (int64_t)X * cny + Y;
-                const T w = wz * wx * wy;
+    const T wz0 = (iz & 1) ? T(0.5) : T(1), wz1 = (iz & 1) ? T(0.5) : T(0);
+    const T wx0 = (ix & 1) ? T(0.5) : T(1), wx1 = (ix & 1) ? T(0.5) : T(0);
+    const T wy0 = (iy & 1) ? T(0.5) : T(1), wy1 = (iy & 1) ? T(0.5) : T(0);
+    const int64_t Zo[2] = {(int64_t)zl * cplane, (int64_t)zh * cplane};
+    const int64_t Xo[2] = {(int64_t)xl * cny, (int64_t)xh * cny};
+    const int Yo[2] = {yl, yh};
+    const T wz[2] = {wz0, wz1}, wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1};
+    T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int64_t J = Zo[a] + Xo[b] + Yo[d];
+                const T w = wz[a] * wx[b] * wy[d];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) acc[c] += w * ec[3 * J + c];
             }
+    const int64_t i = (int64_t)iz * fnx * fny + p;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int64_t dof = 3 * i + c;
@@ -735,7 +837,7 @@ __global__ void k_set_restart(PcgScalars *sc) {
 // x += alpha d; r -= alpha q; partial ||r||^2  (Implementation/SolveMG.m:47-50)
 __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
                                                    const double *__restrict__ d, const double *__restrict__ q,
-                                                   double *partial, const PcgScalars *sc) {
+                                                   double *partial, PcgScalars *sc, int fin) {
     if (sc->done) return;
     const double al = sc->alpha;
     double s = 0.0;
@@ -745,10 +847,9 @@ __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict
         r[i] = ri;
         s += ri * ri;
     }
-    block_reduce_store(s, partial, blockIdx.x);
+    if (fin == FIN_RR) finish_reduce(FIN_RR, s, partial, blockIdx.x, gridDim.x, sc);
+    else block_reduce_store(s, partial, blockIdx.x);
 }
-
-enum { FIN_INIT = 0, FIN_RHO = 1, FIN_ALPHA = 2, FIN_RR = 3 };
 
 // Single-block, fixed-order final reduction and the scalar PCG recurrences.
 template <int MODE>
@@ -771,28 +872,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int np, const double *partial
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double tot = red[0];
-        if (MODE == FIN_INIT) {           // r = f - A x0; nr = ||r||/||f||; it = 1
-            sc->nf = sqrt(red2[0]);
-            sc->rr = tot;
-            sc->nr = sqrt(tot) / sc->nf;
-            sc->it = 1;
-            sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
-        } else if (MODE == FIN_RHO) {     // rho = r' s; beta = rho / gamma2
-            sc->rho = tot;
-            sc->beta = (sc->it == sc->restart) ? 0.0 : tot / sc->gamma;
-        } else if (MODE == FIN_ALPHA) {   // alpha = rho / (d' q)
-            sc->dq = tot;
-            sc->alpha = sc->rho / tot;
-        } else {                          // gamma2 = rho; nr = ||r||/||f||; it++
-            sc->rr = tot;
-            sc->gamma = sc->rho;
-            sc->nr = sqrt(tot) / sc->nf;
-            sc->it = sc->it + 1;
-            sc->done = !(sc->nr > sc->tol && sc->it <= sc->maxiter);
-        }
-    }
+    if (threadIdx.x == 0) fin_update(MODE, red[0], red2[0], sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -818,31 +898,40 @@ __global__ __launch_bounds__(TPB) void k_scatter_full(int64_t ndof, const int64_
     out[d] = q >= 0 ? (T)v[q] : T(0);
 }
 
-// Coarsest-level solve in one launch: x[free[q]] = sum_j Ainv[q][j] b[free[j]],
-// one wave per row, four independent partial sums per lane. Fixed DOFs of x
-// are never written (they stay zero).
+// Coarsest-grid solve x = A^-1 b with the dense FP64 inverse: one block per
+// row, every thread's loads issued together, fixed-order block reduction.
+// Fixed DOFs of x are never written (they stay zero).
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const double *__restrict__ A,
                                                       const int64_t *__restrict__ freeidx, const T *__restrict__ b,
                                                       T *__restrict__ x, const int *done) {
     if (done && *done) return;
-    const int64_t row = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+    const int64_t row = blockIdx.x;
     const double *Ar = A + row * n;
+    // four independent partial sums keep four index -> value -> product
+    // chains in flight per thread
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int64_t j = lane;
-    for (; j + 192 < n; j += 256) {
-        s0 += Ar[j] * (double)b[freeidx[j]];
-        s1 += Ar[j + 64] * (double)b[freeidx[j + 64]];
-        s2 += Ar[j + 128] * (double)b[freeidx[j + 128]];
-        s3 += Ar[j + 192] * (double)b[freeidx[j + 192]];
+    int64_t j = threadIdx.x;
+    for (; j + 3 * TPB < n; j += 4 * TPB) {
+        const int64_t i0 = freeidx[j], i1 = freeidx[j + TPB], i2 = freeidx[j + 2 * TPB], i3 = freeidx[j + 3 * TPB];
+        s0 += Ar[j] * (double)b[i0];
+        s1 += Ar[j + TPB] * (double)b[i1];
+        s2 += Ar[j + 2 * TPB] * (double)b[i2];
+        s3 += Ar[j + 3 * TPB] * (double)b[i3];
     }
-    for (; j < n; j += 64) s0 += Ar[j] * (double)b[freeidx[j]];
+    for (; j < n; j += TPB) s0 += Ar[j] * (double)b[freeidx[j]];
     double s = (s0 + s1) + (s2 + s3);
+    __shared__ double red[TPB / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) x[freeidx[row]] = (T)s;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) t += red[w];
+        x[freeidx[row]] = (T)t;
+    }
 }
 
 __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,

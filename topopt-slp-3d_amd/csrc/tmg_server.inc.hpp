@@ -20,6 +20,8 @@
 
 namespace {
 
+constexpr int SRV_SETUP_THREADS = 4;
+
 struct SrvReq {
     int64_t id = 0;
     tmg_problem prob{};
@@ -87,7 +89,7 @@ struct tmg_server {
     std::vector<hipStream_t> streams;
     std::vector<PcgScalars *> hsc;
     std::vector<char> slot_busy;
-    rocblas_handle rb = nullptr;
+    rocblas_handle rb = nullptr;   // unused (kept zero)
     hipMemPool_t pool = nullptr;
     std::mutex mu;
     std::condition_variable cv;
@@ -96,7 +98,13 @@ struct tmg_server {
     std::unordered_map<int64_t, SrvReq *> all;
     int64_t next_id = 1;
     bool stop = false;
-    std::thread setup_thr, drive_thr;
+    // several setup threads build admitted hierarchies concurrently (host work,
+    // graph capture and the coarse factorization of one request overlap with
+    // the others'); each owns a rocBLAS handle, whose stream binding is
+    // per handle
+    std::vector<std::thread> setup_thr;
+    std::vector<rocblas_handle> rbs;
+    std::thread drive_thr;
     std::chrono::steady_clock::time_point t0;
     std::string err;
     double now() const {
@@ -120,7 +128,7 @@ void srv_release(tmg_server *S, SrvReq *r) {
     S->used -= r->est;
 }
 
-void srv_setup_loop(tmg_server *S) {
+void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
     (void)hipSetDevice(0);
     for (;;) {
         SrvReq *r = nullptr;
@@ -146,7 +154,7 @@ void srv_setup_loop(tmg_server *S) {
         }
         ExtRes ext;
         ext.st = S->streams[r->slot];
-        ext.rb = S->rb;
+        ext.rb = rb;
         ext.h_sc = S->hsc[r->slot];
         ext.pool = S->pool;
         r->prob.freedofs = r->freedofs.data();
@@ -249,7 +257,7 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
         g_setup_err = m;
         for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
         for (PcgScalars *p : S->hsc) (void)hipHostFree(p);
-        if (S->rb) rocblas_destroy_handle(S->rb);
+        for (rocblas_handle h : S->rbs) rocblas_destroy_handle(h);
         if (S->pool) (void)hipMemPoolDestroy(S->pool);
         delete S;
         return TMG_ERR_HIP;
@@ -271,8 +279,13 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
         S->hsc.push_back(p);
     }
     S->slot_busy.assign(max_active, 0);
-    if (rocblas_create_handle(&S->rb) != rocblas_status_success) return bad("rocblas_create_handle failed");
-    S->setup_thr = std::thread(srv_setup_loop, S);
+    const int nsetup = std::min(max_active, SRV_SETUP_THREADS);
+    for (int i = 0; i < nsetup; ++i) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) return bad("rocblas_create_handle failed");
+        S->rbs.push_back(h);
+    }
+    for (int i = 0; i < nsetup; ++i) S->setup_thr.emplace_back(srv_setup_loop, S, S->rbs[i]);
     S->drive_thr = std::thread(srv_drive_loop, S);
     *out = S;
     return TMG_OK;
@@ -381,7 +394,8 @@ void tmg_server_destroy(tmg_server *S) {
         S->stop = true;
     }
     S->cv.notify_all();
-    if (S->setup_thr.joinable()) S->setup_thr.join();
+    for (std::thread &t : S->setup_thr)
+        if (t.joinable()) t.join();
     if (S->drive_thr.joinable()) S->drive_thr.join();
     for (SrvReq *r : S->running) {
         if (r->h) { destroy_all(r->h); delete r->h; }
@@ -389,7 +403,7 @@ void tmg_server_destroy(tmg_server *S) {
     for (auto &kv : S->all) delete kv.second;
     for (hipStream_t st : S->streams) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     for (PcgScalars *p : S->hsc) (void)hipHostFree(p);
-    if (S->rb) rocblas_destroy_handle(S->rb);
+    for (rocblas_handle h : S->rbs) rocblas_destroy_handle(h);
     if (S->pool) (void)hipMemPoolDestroy(S->pool);
     delete S;
 }

@@ -98,13 +98,18 @@ struct tmg_solver {
     int xcd_ebe = 0;                  // the same for the fine-level apply (measured neutral or slower)
     int zchunk64 = 8;                 // z planes per thread of the FP64 fine apply
     bool cube = false;                // cube elements: Ke as ten scalars (KCube)
-    bool ebe_prefetch = true;         // cube fine apply loading the next plane one step ahead
+    bool ebe_prefetch = false;        // cube fine apply loading the next plane one step ahead (measured slower)
     bool fine_stencil = false;        // fine level held as an assembled block stencil (multiresolution)
     KCube<double> kc64{};
     KCube<float> kc32{};
     bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
+    int64_t fused_maxnodes = FUSED_LEVEL_MAXNODES;  // largest stencil level that uses them
     bool fine_r = true;               // fine-level cycle reading the FP64 residual, r's in its post-sweep
+    // PCG reductions finished by the producing kernel's last block instead of
+    // a k_finalize launch: measured much slower (each block's device-scope
+    // fence writes back and invalidates its XCD's L2), so off by default
+    bool fin_fused = false;
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
     // externally owned resources (a server slot): device memory comes from a
@@ -182,6 +187,9 @@ template <> const float *invof<float>(const DevLevel &l) { return l.inv32; }
 
 // Number of blocks an apply of this level launches (partial-sum slots).
 // dbl: the FP64 launch (its z chunk differs from the FP32 one).
+// Grid of the transfer kernels: in-plane blocks x planes.
+inline dim3 plane_grid(int nx, int ny, int nz) { return dim3((unsigned)nblk((int64_t)nx * ny), (unsigned)nz); }
+
 int64_t apply_blocks(tmg_solver *h, int l, bool dbl) {
     const DevLevel &lv = h->lev[l];
     if (l == 0 && !h->fine_stencil) {
@@ -199,6 +207,7 @@ EbeArgs<T, TS> ebe_args(tmg_solver *h) {
     memset(&a, 0, sizeof(a));
     a.nelx = l0.nelx; a.nely = l0.nely; a.nelz = l0.nelz;
     a.zchunk = std::is_same<T, double>::value ? h->zchunk64 : h->zchunk;
+    a.fin = FIN_NONE;
     a.E = Eof<T>(h); a.K0 = K0of<T>(h); a.mask = l0.mask; a.invD = invof<T>(l0);
     a.cnx = h->L > 1 ? h->lev[1].nx : 1;
     a.cny = h->L > 1 ? h->lev[1].ny : 1;
@@ -210,7 +219,7 @@ void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
     const DevLevel &lv = h->lev[0];
     dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + a.zchunk - 1) / a.zchunk);
 #define EBE_ARGS a.nelx, a.nely, a.nelz, a.zchunk, a.x, a.E, a.K0, a.mask, a.b, a.invD, a.r, a.ec, a.cnx, \
-                 a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KCof<T>(h), h->xcd_ebe
+                 a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KCof<T>(h), h->xcd_ebe, a.fin
     if constexpr (SRC == SRC_X) {
         if (h->nsub == 8) {
             hipLaunchKernelGGL((k_ebe<T, TS, SRC, EPI, false, 8>), grid, dim3(TPB), 0, h->st, EBE_ARGS);
@@ -260,7 +269,7 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
-    hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)((h->nc + 3) / 4)), dim3(TPB), 0, h->st, h->nc, h->Ainv,
+    hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc, h->Ainv,
                        lc.d_free, b, x, done);
 }
 
@@ -295,12 +304,12 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
     T *xa = (T *)lv.x, *xb = (T *)lv.t;
     const int nu1 = h->mg.smoothiter1, nu2 = h->mg.smoothiter2;
     // small stencil levels are launch-bound: fold the sweeps into the applies there
-    if (!x0 && l > 0 && nu1 == 1 && nu2 == 1 && h->fused_coarse && lv.nn <= FUSED_LEVEL_MAXNODES) {
+    if (!x0 && l > 0 && nu1 == 1 && nu2 == 1 && h->fused_coarse && lv.nn <= h->fused_maxnodes) {
         // stencil level, V(1,1): rh = b - A (M\b); then s = w + M\(b - A w), w = M\b + P eH
         StArgs<T> a = st_args<T>(h, l);
         a.b = b; a.out = xb; a.done = done;
         hipLaunchKernelGGL((k_stencil<T, SRC_JZ, EPI_RESID>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
-        hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
+        hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
                            lc.nz, xb, lc.mask, (T *)lc.b, done);
         T *ec = coarse_correction<T>(h, l, done);
         StArgs<T> c = st_args<T>(h, l);
@@ -327,10 +336,10 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
         hipLaunchKernelGGL((k_fill<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, T(0), xa);
     }
     apply<T, EPI_RESID>(h, l, xa, b, xb, nullptr, done);                          // rh = b - A x
-    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st,    // rH = P' rh
+    hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st,    // rH = P' rh
                        lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, xb, lc.mask, (T *)lc.b, done);
     T *ec = coarse_correction<T>(h, l, done);
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st,  // x += P eH
+    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(lv.nx, lv.ny, lv.nz), dim3(TPB), 0, h->st,  // x += P eH
                        lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, (const T *)ec, lv.mask, xa, 1, done);
     for (int s = 0; s < nu2; ++s) {
         apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
@@ -353,7 +362,7 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
     EbeArgs<T, T> a = ebe_args<T, T>(h);
     a.r = r; a.done = done; a.out = (T *)l0.t;
     launch_ebe<T, T, SRC_JZ, EPI_RESIDR>(h, a);
-    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(l1.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+    hipLaunchKernelGGL((k_restrict<T>), plane_grid(l1.nx, l1.ny, l1.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
                        l1.nz, (const T *)l0.t, l1.mask, (T *)l1.b, done);
     T *ec = coarse_correction<T>(h, 0, done);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
@@ -382,13 +391,14 @@ T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done) {
     EbeArgs<T, T> a = ebe_args<T, T>(h);
     a.x = xa; a.r = r; a.out = xb; a.done = done;
     launch_ebe<T, T, SRC_X, EPI_RESIDR>(h, a);
-    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(l1.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+    hipLaunchKernelGGL((k_restrict<T>), plane_grid(l1.nx, l1.ny, l1.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
                        l1.nz, (const T *)xb, l1.mask, (T *)l1.b, done);
     T *ec = coarse_correction<T>(h, 0, done);
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
+    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(l0.nx, l0.ny, l0.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
                        l1.nz, (const T *)ec, l0.mask, xa, 1, done);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.x = xa; b.r = r; b.out = xb; b.partial = h->partial; b.done = done;
+    if (h->fin_fused) { b.sc = h->d_sc; b.fin = FIN_RHO; }       // rho = r's, beta: in the last block
     launch_ebe<T, T, SRC_X, EPI_POSTR>(h, b);
     return xb;
 }
@@ -442,6 +452,7 @@ void pcg_iteration(tmg_solver *h, int parity) {
     const int nb = h->npartial;
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), nb);
     const int napply = (int)apply_blocks(h, 0, true);
+    const int finrr = h->fin_fused ? FIN_RR : FIN_NONE;
     int nrho = 0;
     precondition(h, h->pr, &s64, &s32, done, &nrho);
     if (fused_ok(h)) {
@@ -452,11 +463,14 @@ void pcg_iteration(tmg_solver *h, int parity) {
         if (s32) direction_apply<float>(h, s32, dold, dnew, done);
         else direction_apply<double>(h, s64, dold, dnew, done);
         hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
-        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, dnew, h->pq, h->partial, h->d_sc);
+        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, dnew, h->pq, h->partial, h->d_sc,
+                           finrr);
     } else {
         if (nrho > 0) {
-            // rho = r' s from the post-sweep's partial sums
-            hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, nrho, h->partial, 0, nullptr, h->d_sc);
+            // rho = r' s from the post-sweep's partial sums (finished there when fin_fused)
+            if (!h->fin_fused)
+                hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, nrho, h->partial, 0, nullptr,
+                                   h->d_sc);
         } else {
             if (s32) hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
             else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
@@ -464,11 +478,21 @@ void pcg_iteration(tmg_solver *h, int parity) {
         }
         if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
         else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
-        apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);          // q = A d, d'q
-        hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
-        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial, h->d_sc);
+        if (h->fin_fused && !h->fine_stencil) {
+            // q = A d, d'q, alpha in the apply's last block
+            EbeArgs<double, double> a = ebe_args<double, double>(h);
+            a.x = h->pd; a.out = h->pq; a.partial = h->partial; a.done = done; a.sc = h->d_sc; a.fin = FIN_ALPHA;
+            launch_ebe<double, double, SRC_X, EPI_DOT>(h, a);
+        } else {
+            apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);      // q = A d, d'q
+            hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr,
+                               h->d_sc);
+        }
+        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial,
+                           h->d_sc, finrr);
     }
-    hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+    if (!h->fin_fused)
+        hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
 }
 
 int check_launch(tmg_solver *h) {
@@ -894,6 +918,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_NO_GRAPH")) h->use_graph = ev[0] == '0';
     h->escalate = !getenv_flag("TMG_NO_ESCALATE");
     if (const char *ev = getenv("TMG_FINE_R")) h->fine_r = ev[0] != '0';
+    if (const char *ev = getenv("TMG_FIN_FUSED")) h->fin_fused = ev[0] != '0';
+    if (const char *ev = getenv("TMG_FUSED_MAXNODES")) h->fused_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
@@ -1177,7 +1203,7 @@ int tmg_prolong(tmg_solver *h, int level, const double *xc, double *xf) {
     TRY(dalloc(h, &df, lf.ndof));
     int rc = upload_free(h, level, xc, dc);
     if (!rc) {
-        hipLaunchKernelGGL((k_prolong_add<double>), dim3(nblk(lf.nn)), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
+        hipLaunchKernelGGL((k_prolong_add<double>), plane_grid(lf.nx, lf.ny, lf.nz), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
                            lc.ny, lc.nz, dc, lf.mask, df, 0, nullptr);
         rc = check_launch(h);
     }
@@ -1195,7 +1221,7 @@ int tmg_restrict(tmg_solver *h, int level, const double *xf, double *xc) {
     TRY(dalloc(h, &df, lf.ndof));
     int rc = upload_free(h, level - 1, xf, df);
     if (!rc) {
-        hipLaunchKernelGGL((k_restrict<double>), dim3(nblk(lc.nn)), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
+        hipLaunchKernelGGL((k_restrict<double>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
                            lc.ny, lc.nz, df, lc.mask, dc, nullptr);
         rc = check_launch(h);
     }
