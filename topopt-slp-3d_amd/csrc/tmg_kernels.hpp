@@ -888,6 +888,20 @@ __global__ __launch_bounds__(TPB) void k_gather(int64_t n, const int64_t *__rest
     if (q < n) out[q] = (double)v[idx[q]];
 }
 
+// Finest-level index maps from the 1-based free-DOF list (strictly
+// increasing, validated on the host): mask, full -> compact index and
+// compact -> full. mask and cidx are pre-filled with 0 and -1.
+__global__ __launch_bounds__(TPB) void k_level_index(int64_t nfree, const int64_t *__restrict__ fd1,
+                                                     uint8_t *__restrict__ mask, int64_t *__restrict__ cidx,
+                                                     int64_t *__restrict__ dfree) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nfree) return;
+    const int64_t d = fd1[i] - 1;
+    mask[d] = 1;
+    cidx[d] = i;
+    dfree[i] = d;
+}
+
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_scatter_full(int64_t ndof, const int64_t *__restrict__ cidx,
                                                       const double *__restrict__ v, T *__restrict__ out, const int *done) {

@@ -642,25 +642,49 @@ int solve_resident(tmg_solver *h, int *it, double *relres) {
 // ---------------------------------------------------------------------------
 // setup / refresh
 // ---------------------------------------------------------------------------
-int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask) {
+// fd1: the finest level's 1-based free-DOF list (l = 0), whose index maps
+// are then built on the device; coarse levels build theirs on the host from
+// the mask (1/8 of the size per level).
+int alloc_level(tmg_solver *h, int l, const std::vector<uint8_t> &mask, const int64_t *fd1 = nullptr,
+                int64_t nfree1 = 0) {
     DevLevel &lv = h->lev[l];
     lv.nx = lv.nelx + 1; lv.ny = lv.nely + 1; lv.nz = lv.nelz + 1;
     lv.nn = (int64_t)lv.nx * lv.ny * lv.nz;
     lv.ne = (int64_t)lv.nelx * lv.nely * lv.nelz;
     lv.ndof = 3 * lv.nn;
     lv.free_full.clear();
-    std::vector<int64_t> cidx(lv.ndof, -1);
-    for (int64_t d = 0; d < lv.ndof; ++d)
-        if (mask[d]) { cidx[d] = (int64_t)lv.free_full.size(); lv.free_full.push_back(d); }
-    lv.nfree = (int64_t)lv.free_full.size();
-    TRY(dalloc(h, &lv.mask, lv.ndof));
-    TRY(dalloc(h, &lv.d_free, lv.nfree));
-    TRY(dalloc(h, &lv.d_cidx, lv.ndof));
-    HIPX(hipMemcpyAsync(lv.mask, mask.data(), lv.ndof, hipMemcpyHostToDevice, h->st));
-    if (lv.nfree)
-        HIPX(hipMemcpyAsync(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice, h->st));
-    HIPX(hipMemcpyAsync(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice, h->st));
-    HIPX(hipStreamSynchronize(h->st));  // host staging vectors go out of scope
+    if (fd1) {
+        lv.nfree = nfree1;
+        lv.free_full.resize((size_t)nfree1);
+        for (int64_t i = 0; i < nfree1; ++i) lv.free_full[i] = fd1[i] - 1;
+        int64_t *dfd1 = nullptr;
+        TRY(dalloc(h, &lv.mask, lv.ndof));
+        TRY(dalloc(h, &lv.d_free, lv.nfree));
+        TRY(dalloc(h, &lv.d_cidx, lv.ndof));
+        TRY(dalloc(h, &dfd1, std::max<int64_t>(nfree1, 1)));
+        hipLaunchKernelGGL((k_fill<uint8_t>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, (uint8_t)0, lv.mask);
+        hipLaunchKernelGGL((k_fill<int64_t>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, (int64_t)-1, lv.d_cidx);
+        if (nfree1) {
+            HIPX(hipMemcpyAsync(dfd1, fd1, sizeof(int64_t) * nfree1, hipMemcpyHostToDevice, h->st));
+            hipLaunchKernelGGL(k_level_index, dim3(nblk(nfree1)), dim3(TPB), 0, h->st, nfree1, dfd1, lv.mask, lv.d_cidx,
+                               lv.d_free);
+        }
+        HIPX(hipStreamSynchronize(h->st));
+        dfree(h, dfd1);
+    } else {
+        std::vector<int64_t> cidx(lv.ndof, -1);
+        for (int64_t d = 0; d < lv.ndof; ++d)
+            if (mask[d]) { cidx[d] = (int64_t)lv.free_full.size(); lv.free_full.push_back(d); }
+        lv.nfree = (int64_t)lv.free_full.size();
+        TRY(dalloc(h, &lv.mask, lv.ndof));
+        TRY(dalloc(h, &lv.d_free, lv.nfree));
+        TRY(dalloc(h, &lv.d_cidx, lv.ndof));
+        HIPX(hipMemcpyAsync(lv.mask, mask.data(), lv.ndof, hipMemcpyHostToDevice, h->st));
+        if (lv.nfree)
+            HIPX(hipMemcpyAsync(lv.d_free, lv.free_full.data(), sizeof(int64_t) * lv.nfree, hipMemcpyHostToDevice, h->st));
+        HIPX(hipMemcpyAsync(lv.d_cidx, cidx.data(), sizeof(int64_t) * lv.ndof, hipMemcpyHostToDevice, h->st));
+        HIPX(hipStreamSynchronize(h->st));  // host staging vectors go out of scope
+    }
     // V-cycle work vectors are sized for FP64 in both modes: a mixed handle
     // runs its FP64 cycle in them when a solve escalates (solve_check)
     TRY(dalloc(h, (char **)&lv.b, (int64_t)(lv.ndof * sizeof(double))));
@@ -978,7 +1002,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                     }
             mask.swap(cm);
         }
-        int rc = alloc_level(h, l, mask);
+        int rc = l == 0 ? alloc_level(h, l, mask, prob->freedofs, prob->nfree) : alloc_level(h, l, mask);
         if (rc) return fail(rc);
     }
     h->nc = h->lev[h->L - 1].nfree;
