@@ -145,6 +145,7 @@ struct EbeArgs {
     const double *dold;    // previous direction (SRC_DIR)
     double *dnew;          // new direction (EPI_DIRDOT)
     PcgScalars *sc;        // beta, it (SRC_DIR); reduction finish (fin >= 0)
+    double dbeta;          // SRC_DIR: beta, 0 on the first iteration since a restart (d = s)
     int fin;               // FIN_* applied by the last block of a DOT-type epilogue
     T *out;
     double *partial;       // per-block partial sums (DOT, POSTR, DIRDOT)
@@ -189,13 +190,22 @@ __device__ inline void ebe_source(const EbeArgs<T, TS> &a, int jx, int jy, int j
 #pragma unroll
             for (int c = 0; c < 3; ++c) v[c] += a.mask[3 * node + c] ? T(0.125) * pe[c] : T(0);
         }
-    } else {  // SRC_DIR (T = double)
-        const bool first = a.sc->it == a.sc->restart;
-        const double beta = a.sc->beta;
+    } else {  // SRC_DIR (T = double); dbeta read once per thread (ebe_dir_scalars)
+        // branch-free: a select between the two forms compiles to a branch
+        // per value with its own load and wait. dbeta = 0 on the first
+        // iteration since a restart, where d_old is finite (zeroed at the
+        // start of a solve, else the previous direction).
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            v[c] = first ? (T)a.s[3 * node + c] : (T)(beta * a.dold[3 * node + c] + (double)a.s[3 * node + c]);
+        for (int c = 0; c < 3; ++c) v[c] = (T)fma(a.dbeta, a.dold[3 * node + c], (double)a.s[3 * node + c]);
     }
+}
+
+// The direction update's scalars, read once before any store of the kernel
+// (read per node, they would be re-loaded after every store that may alias).
+template <typename T, typename TS, int SRC>
+__device__ inline void ebe_dir_scalars(EbeArgs<T, TS> &a) {
+    a.dbeta = 0.0;
+    if (SRC == SRC_DIR) a.dbeta = a.sc->it == a.sc->restart ? 0.0 : a.sc->beta;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
@@ -266,6 +276,7 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
     a.nelx = nelx_; a.nely = nely_; a.nelz = nelz_; a.zchunk = zchunk_; a.x = x_; a.E = E_; a.K0 = K0_;
     a.mask = mask_; a.b = b_; a.invD = invD_; a.r = r_; a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.s = s_;
     a.dold = dold_; a.dnew = dnew_; a.sc = sc_; a.out = out_; a.partial = partial_; a.done = done_; a.fin = fin_;
+    ebe_dir_scalars<T, TS, SRC>(a);
     const int nelx = a.nelx, nely = a.nely, nelz = a.nelz;
     const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
     const int64_t plane = (int64_t)nx * ny;
@@ -536,6 +547,263 @@ __global__ __launch_bounds__(TPB) void k_ebe(int nelx_, int nely_, int nelz_, in
         else
             block_reduce_store(dacc, a.partial, bz * gridDim.x + bx);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fine-level operator in spectral element form.
+//
+// The Q1 element matrix of a box is invariant under reflection of the box in
+// each coordinate plane (a reflection in direction k maps local node a to
+// a ^ e_k and flips displacement component k), so
+//     Ke[3a+c][3b+d] = s_a(c) s_b(d) g_cd(a ^ b),   s_a(c) = (-1)^(bit e_c of a),
+// with e_x = 2, e_y = 1, e_z = 4 the bits of a = dy + 2dx + 4dz. The 8-point
+// Walsh-Hadamard transform H (H[k][b] = (-1)^popcount(k & b)) diagonalises
+// every dyadic convolution, and the sign factors only shift frequencies:
+//     (H y_c)(k) = sum_d G[c][d][k] (H x_d)(k ^ e_c ^ e_d),
+//     G[c][d][k] = (H g_cd)(k ^ e_c) / 8,       y_c = H (H y_c).
+// Only 45 of the 72 coefficients G are nonzero (WHT_NZ), so an element
+// product costs two 24-value transforms and 45 multiply-adds instead of the
+// 576 of a dense Ke. The transforms are separable: the in-plane 2x2 part of a
+// node plane is formed once and serves the element layers below and above it,
+// and the two layers' contributions to a node plane are summed before the
+// plane's inverse transform.
+//
+// Work split: one thread per element column (ex, ey) of a TX x TY tile
+// (TX * TY = TPB, y fastest), streaming the element layers of a z chunk.
+// Node (ex, ey) sums corner (dx,dy) = (0,0) of its own element column with
+// corners (0,1), (1,0), (1,1) of the columns at (ex, ey-1), (ex-1, ey) and
+// (ex-1, ey-1), exchanged through LDS. Tiles overlap by one column in x and
+// y; the first row and column of a tile only compute (halo). Elements
+// outside the grid have modulus 0 and node loads are clamped into the grid,
+// so their contributions are exact zeros. Every sum has a fixed order, so
+// the result is bitwise reproducible.
+// ---------------------------------------------------------------------------
+constexpr unsigned char WHT_NZ[3][3] = {{0xFE, 0x66, 0x3C}, {0x66, 0xFE, 0x5A}, {0x3C, 0x5A, 0xFE}};
+constexpr int WHT_BIT[3] = {2, 1, 4};
+constexpr int WHT_NCOEF = 45;
+__host__ __device__ constexpr int wht_slot(int c, int d, int k) {
+    int s = 0;
+    for (int i = 0; i < 3 * c + d; ++i)
+        for (int kk = 0; kk < 8; ++kk) s += (WHT_NZ[i / 3][i % 3] >> kk) & 1;
+    for (int kk = 0; kk < k; ++kk) s += (WHT_NZ[c][d] >> kk) & 1;
+    return s;
+}
+template <typename T>
+struct KWht {
+    T g[WHT_NCOEF];
+};
+// Smallest z chunk of the spectral apply whose grid fits in one resident
+// round (cap blocks), else the chunk minimising rounds x (layers per block).
+inline int wht_zchunk(int64_t tiles, int nz, int64_t cap) {
+    int best = 2;
+    int64_t bcost = -1;
+    for (int zc = 2; zc <= 32; ++zc) {
+        const int64_t blocks = tiles * ((nz + zc - 1) / zc);
+        const int64_t rounds = (blocks + cap - 1) / (cap > 0 ? cap : 1);
+        const int64_t cost = rounds * (zc + 1);
+        if (bcost < 0 || cost < bcost) { bcost = cost; best = zc; }
+    }
+    return best;
+}
+
+// 2x2 transform of a node-plane patch, v[b] with b = dy + 2dx.
+template <typename T>
+__device__ inline void wht4(const T (&v)[4], T (&P)[4]) {
+    const T s01 = v[0] + v[1], d01 = v[0] - v[1], s23 = v[2] + v[3], d23 = v[2] - v[3];
+    P[0] = s01 + s23;
+    P[1] = d01 + d23;
+    P[2] = s01 - s23;
+    P[3] = d01 - d23;
+}
+
+// tyl: log2 of the tile's y extent TY (TX = TPB / TY); ntx: tiles in x;
+// kw_: the WHT_NCOEF coefficients G in slot order. PF: the loads of the next
+// node plane are issued before the current layer's arithmetic.
+template <typename T, typename TS, int SRC, int EPI, bool PF>
+__global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_, int zchunk_, int tyl_, int ntx_,
+                                                 const T *__restrict__ x_, const T *__restrict__ E_,
+                                                 const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
+                                                 const T *__restrict__ invD_, const double *__restrict__ r_,
+                                                 const T *__restrict__ ec_, int cnx_, int cny_,
+                                                 const TS *__restrict__ s_, const double *__restrict__ dold_,
+                                                 double *__restrict__ dnew_, PcgScalars *__restrict__ sc_,
+                                                 T *__restrict__ out_, double *__restrict__ partial_,
+                                                 const int *__restrict__ done_, const T *__restrict__ kw_, int xcd_) {
+    if (done_ && *done_) return;
+    EbeArgs<T, TS> a;
+    a.nelx = nelx_; a.nely = nely_; a.nelz = nelz_; a.zchunk = zchunk_; a.x = x_; a.E = E_; a.K0 = nullptr;
+    a.mask = mask_; a.b = b_; a.invD = invD_; a.r = r_; a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.s = s_;
+    a.dold = dold_; a.dnew = dnew_; a.sc = sc_; a.out = out_; a.partial = partial_; a.done = done_; a.fin = FIN_NONE;
+    ebe_dir_scalars<T, TS, SRC>(a);
+    const int nelx = nelx_, nely = nely_, nelz = nelz_;
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const int64_t plane = (int64_t)nx * ny;
+    const int TY = 1 << tyl_, TX = TPB >> tyl_;
+    const int t = (int)threadIdx.x;
+    const int tx = t >> tyl_, ty = t & (TY - 1);
+    // logical block: tile (bt) and z chunk (bz); with xcd_ the blocks of one
+    // XCD take a contiguous range, so neighbouring tiles share an L2
+    int bt = (int)blockIdx.x, bz = (int)blockIdx.y;
+    if (xcd_) {
+        const int64_t lb = xcd_block((int64_t)blockIdx.y * gridDim.x + blockIdx.x, (int64_t)gridDim.x * gridDim.y);
+        bz = (int)(lb / gridDim.x);
+        bt = (int)(lb - (int64_t)bz * gridDim.x);
+    }
+    const int bxx = bt % ntx_, byy = bt / ntx_;
+    const int ix = bxx * (TX - 1) + tx - 1, iy = byy * (TY - 1) + ty - 1;   // element column = its (0,0) node
+    const bool out_node = tx >= 1 && ty >= 1 && ix < nx && iy < ny;
+    const bool ecol = ix >= 0 && ix < nelx && iy >= 0 && iy < nely;
+    const int z0 = bz * zchunk_;
+    const int z1 = min(z0 + zchunk_, nz);
+    // patch nodes b = dy + 2dx, clamped into the grid; in-plane offsets fit 32 bits
+    int px[2], py[2];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        px[o] = min(max(ix + o, 0), nx - 1);
+        py[o] = min(max(iy + o, 0), ny - 1);
+    }
+    int poff[4];
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb) poff[bb] = px[bb >> 1] * ny + py[bb & 1];
+    const int ecol_idx = ecol ? ix * nely + iy : 0;
+    const int own = poff[0];
+
+    __shared__ T xch[2][3][3][TPB];   // [buffer][corner - 1][component][thread]
+
+    T Pb[3][4], Pt[3][4];            // plane transforms below / above the current layer
+    T wb[3], wt[3];                  // source values of node (ix, iy) on those planes
+    T nv[4][3];                      // raw source values of the next plane
+    auto load_raw = [&](int zz) {
+        const int jz = min(max(zz, 0), nz - 1);
+        const int64_t zb = (int64_t)jz * plane;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) ebe_source<T, TS, SRC>(a, px[bb >> 1], py[bb & 1], jz, zb + poff[bb], nv[bb]);
+    };
+    auto transform = [&](T (&P)[3][4], T (&w)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T vc[4] = {nv[0][c], nv[1][c], nv[2][c], nv[3][c]};
+            wht4(vc, P[c]);
+            w[c] = nv[0][c];
+        }
+    };
+    auto modulus = [&](int ez) {
+        const bool ok = ecol && ez >= 0 && ez < nelz;
+        const T e = E_[(int64_t)min(max(ez, 0), nelz - 1) * nelx * nely + ecol_idx];
+        return ok ? e : T(0);
+    };
+    // coefficients through the constant address space (scalar loads); the
+    // base is re-materialised per component so that only a third of them
+    // occupies SGPRs at a time
+    typedef const __attribute__((address_space(4))) T *cptr;
+    cptr kp = (cptr)kw_;
+
+    load_raw(z0 - 1);
+    transform(Pt, wt);
+    if (PF) load_raw(z0);
+    T Epre = modulus(z0 - 1);
+    T Qtp[3][4];                     // top-plane spectra of the previous layer
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Qtp[c][k] = T(0);
+    double dacc = 0.0;
+    for (int L = z0 - 1; L < z1; ++L) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            wb[c] = wt[c];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Pb[c][k] = Pt[c][k];
+        }
+        if (!PF) load_raw(L + 1);
+        transform(Pt, wt);               // plane L + 1
+        if (PF) load_raw(L + 2);         // in flight during this layer's arithmetic
+        const T Ee = Epre;
+        Epre = modulus(L + 1);
+        // element spectrum x^(k), k = kxy + 4 kz (x^(0) is never used)
+        T xh[3][8];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xh[d][k] = Pb[d][k] + Pt[d][k];
+                xh[d][k + 4] = Pb[d][k] - Pt[d][k];
+            }
+        // y^ = E G x^ (y^(0) = 0: translations are in the null space)
+        T yh[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c > 0) asm volatile("" : "+s"(kp) : "v"(yh[c - 1][7]));
+            yh[c][0] = T(0);
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+                T acc = T(0);
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    if ((WHT_NZ[c][d] >> k) & 1)
+                        acc = fma(kp[wht_slot(c, d, k)], xh[d][k ^ WHT_BIT[c] ^ WHT_BIT[d]], acc);
+                yh[c][k] = Ee * acc;
+            }
+        }
+        asm volatile("" : "+s"(kp) : "v"(yh[2][7]));
+        // inverse in z: bottom / top plane spectra; the bottom one completes
+        // node plane L together with the previous layer's top one
+        T cv[3][4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T S[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                S[k] = Qtp[c][k] + (yh[c][k] + yh[c][k + 4]);
+                Qtp[c][k] = yh[c][k] - yh[c][k + 4];
+            }
+            wht4(S, cv[c]);
+        }
+        if (L < z0) continue;   // uniform across the block: no barrier is skipped unevenly
+        const int buf = (L - z0) & 1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xch[buf][j][c][t] = cv[c][j + 1];
+        __syncthreads();
+        if (out_node) {
+            const int64_t node = (int64_t)L * plane + own;   // 64-bit: full-grid DOF index
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T y = cv[c][0] + xch[buf][0][c][t - 1] + xch[buf][1][c][t - TY] + xch[buf][2][c][t - TY - 1];
+                const int64_t dof = 3 * node + c;
+                const bool m = a.mask[dof] != 0;
+                const T w = wb[c];
+                if (EPI == EPI_PLAIN) {
+                    a.out[dof] = m ? y : T(0);
+                } else if (EPI == EPI_RESID) {
+                    a.out[dof] = m ? (a.b[dof] - y) : T(0);
+                } else if (EPI == EPI_JACOBI) {
+                    a.out[dof] = m ? (w + a.invD[dof] * (a.b[dof] - y)) : T(0);
+                } else if (EPI == EPI_DOT) {
+                    const T v = m ? y : T(0);
+                    a.out[dof] = v;
+                    dacc += (double)w * (double)v;
+                } else if (EPI == EPI_RESIDR) {
+                    a.out[dof] = m ? ((T)a.r[dof] - y) : T(0);
+                } else if (EPI == EPI_POSTR) {
+                    const double rv = a.r[dof];
+                    const T v = m ? (w + a.invD[dof] * ((T)rv - y)) : T(0);
+                    a.out[dof] = v;
+                    dacc += rv * (double)v;
+                } else {  // EPI_DIRDOT
+                    const T v = m ? y : T(0);
+                    a.out[dof] = v;
+                    a.dnew[dof] = (double)w;
+                    dacc += (double)w * (double)v;
+                }
+            }
+        }
+    }
+    // partial slots in logical (spatial) order: the reduction order does
+    // not depend on the block mapping
+    if (EPI == EPI_DOT || EPI == EPI_POSTR || EPI == EPI_DIRDOT)
+        block_reduce_store(dacc, a.partial, (int64_t)bz * gridDim.x + bt);
 }
 
 // ---------------------------------------------------------------------------

@@ -102,6 +102,17 @@ struct tmg_solver {
     bool fine_stencil = false;        // fine level held as an assembled block stencil (multiresolution)
     KCube<double> kc64{};
     KCube<float> kc32{};
+    // spectral element form of the fine apply (k_ebe_wht): coupling
+    // coefficients, tile shape (TY = 1 << wht_tyl, TX = TPB / TY) and tiles
+    bool wht = false;
+    bool wht_pf64 = true;             // spectral apply with the next plane prefetched: FP64 (faster)
+    bool wht_pf32 = false;            // and FP32 (slower: the extra registers cost a wave)
+    int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
+    KWht<double> kw64{};
+    KWht<float> kw32{};
+    double *kw64_d = nullptr;         // the same coefficients on the device
+    float *kw32_d = nullptr;
+    int wht_tyl = 4, wht_ntx = 1, wht_nty = 1;
     bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
     int64_t fused_maxnodes = FUSED_LEVEL_MAXNODES;  // largest stencil level that uses them
@@ -193,7 +204,7 @@ inline dim3 plane_grid(int nx, int ny, int nz) { return dim3((unsigned)nblk((int
 int64_t apply_blocks(tmg_solver *h, int l, bool dbl) {
     const DevLevel &lv = h->lev[l];
     if (l == 0 && !h->fine_stencil) {
-        int64_t pb = nblk((int64_t)lv.nx * lv.ny);
+        int64_t pb = h->wht ? (int64_t)h->wht_ntx * h->wht_nty : nblk((int64_t)lv.nx * lv.ny);
         const int zc = dbl ? h->zchunk64 : h->zchunk;
         return pb * ((lv.nz + zc - 1) / zc);
     }
@@ -214,9 +225,25 @@ EbeArgs<T, TS> ebe_args(tmg_solver *h) {
     return a;
 }
 
+template <typename T>
+const T *KWof(tmg_solver *h);
+template <> const double *KWof<double>(tmg_solver *h) { return h->kw64_d; }
+template <> const float *KWof<float>(tmg_solver *h) { return h->kw32_d; }
+
 template <typename T, typename TS, int SRC, int EPI>
 void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
     const DevLevel &lv = h->lev[0];
+    if (h->wht && h->nsub == 1 && a.fin < 0) {
+        dim3 gw((unsigned)(h->wht_ntx * h->wht_nty), (unsigned)((lv.nz + a.zchunk - 1) / a.zchunk));
+#define WHT_ARGS a.nelx, a.nely, a.nelz, a.zchunk, h->wht_tyl, h->wht_ntx, a.x, a.E, a.mask, a.b, a.invD, a.r, a.ec, \
+                 a.cnx, a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KWof<T>(h), h->xcd_wht
+        if (std::is_same<T, double>::value ? h->wht_pf64 : h->wht_pf32)
+            hipLaunchKernelGGL((k_ebe_wht<T, TS, SRC, EPI, true>), gw, dim3(TPB), 0, h->st, WHT_ARGS);
+        else
+            hipLaunchKernelGGL((k_ebe_wht<T, TS, SRC, EPI, false>), gw, dim3(TPB), 0, h->st, WHT_ARGS);
+#undef WHT_ARGS
+        return;
+    }
     dim3 grid(nblk((int64_t)lv.nx * lv.ny), (lv.nz + a.zchunk - 1) / a.zchunk);
 #define EBE_ARGS a.nelx, a.nely, a.nelz, a.zchunk, a.x, a.E, a.K0, a.mask, a.b, a.invD, a.r, a.ec, a.cnx, \
                  a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KCof<T>(h), h->xcd_ebe, a.fin
@@ -712,7 +739,7 @@ void destroy_all(tmg_solver *h) {
         dfree(h, lv.mask); dfree(h, lv.S64); dfree(h, lv.S32); dfree(h, lv.inv64); dfree(h, lv.inv32);
         dfree(h, lv.b); dfree(h, lv.x); dfree(h, lv.t); dfree(h, lv.d_free); dfree(h, lv.d_cidx);
     }
-    dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
+    dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
     dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
@@ -1050,9 +1077,73 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
             for (int k = 0; k < 10; ++k) { h->kc64.v[k] = v[k]; h->kc32.v[k] = (float)v[k]; }
         }
     }
+    // spectral element form: g_cd(o) = s_o(d) Ke[c][3o+d], G = shifted
+    // transform of g / 8; accepted only if Ke has the reflection structure
+    // (every entry reproduced) and G vanishes outside the WHT_NZ pattern
+    h->wht = false;
+    if (h->nsub == 1 && !getenv_flag("TMG_EBE_GENERIC") && !(getenv("TMG_EBE_WHT") && getenv("TMG_EBE_WHT")[0] == '0')) {
+        const int bitc[3] = {1, 0, 2};
+        auto sgn = [&](int a, int c) { return ((a >> bitc[c]) & 1) ? -1.0 : 1.0; };
+        double kmax = 0.0;
+        for (int i = 0; i < 576; ++i) kmax = std::max(kmax, std::fabs(kop[i]));
+        double g[3][3][8];
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int o = 0; o < 8; ++o) g[c][d][o] = sgn(o, d) * kop[c * 24 + 3 * o + d];
+        bool ok = true;
+        for (int a = 0; ok && a < 8; ++a)
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 8; ++b)
+                    for (int d = 0; d < 3; ++d)
+                        ok = ok && std::fabs(sgn(a, c) * sgn(b, d) * g[c][d][a ^ b] - kop[(3 * a + c) * 24 + 3 * b + d]) <=
+                                       1e-13 * kmax;
+        double G[3][3][8];
+        double gmax = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 8; ++k) {
+                    const int m = k ^ (1 << bitc[c]);
+                    double sum = 0.0;
+                    for (int o = 0; o < 8; ++o) sum += (__builtin_popcount(m & o) & 1 ? -1.0 : 1.0) * g[c][d][o];
+                    G[c][d][k] = sum / 8.0;
+                    gmax = std::max(gmax, std::fabs(G[c][d][k]));
+                }
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 8; ++k) {
+                    if ((WHT_NZ[c][d] >> k) & 1) {
+                        h->kw64.g[wht_slot(c, d, k)] = G[c][d][k];
+                        h->kw32.g[wht_slot(c, d, k)] = (float)G[c][d][k];
+                    } else {
+                        ok = ok && std::fabs(G[c][d][k]) <= 1e-13 * gmax;
+                    }
+                }
+        if (ok) {
+            h->wht = true;
+            if (const char *ev = getenv("TMG_WHT_PREFETCH")) h->wht_pf32 = h->wht_pf64 = ev[0] != '0';
+            if (const char *ev = getenv("TMG_XCD_WHT")) h->xcd_wht = ev[0] != '0';
+            // tile shape: TY in {8, 16, 32} with the fewest tiles per node plane
+            int64_t best = -1;
+            for (int tyl = 3; tyl <= 5; ++tyl) {
+                const int TY = 1 << tyl, TX = TPB / TY;
+                const int ntx = (l0.nx + TX - 2) / (TX - 1), nty = (l0.ny + TY - 2) / (TY - 1);
+                const int64_t nt = (int64_t)ntx * nty;
+                if (best < 0 || nt < best || (nt == best && tyl == 4)) {
+                    best = nt; h->wht_tyl = tyl; h->wht_ntx = ntx; h->wht_nty = nty;
+                }
+            }
+            if (const char *ev = getenv("TMG_WHT_TYL")) {
+                const int tyl = std::max(3, std::min(atoi(ev), 5)), TY = 1 << tyl, TX = TPB / TY;
+                h->wht_tyl = tyl;
+                h->wht_ntx = (l0.nx + TX - 2) / (TX - 1);
+                h->wht_nty = (l0.ny + TY - 2) / (TY - 1);
+            }
+        }
+    }
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
     if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
+        (rc = dalloc(h, &h->kw64_d, WHT_NCOEF)) || (rc = dalloc(h, &h->kw32_d, WHT_NCOEF)) ||
         (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
@@ -1070,6 +1161,27 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->zchunk = std::max(1, std::min(zc, 16));
         int zc64 = (int)((l0.nz * pb + 2047) / 2048);
         h->zchunk64 = std::max(1, std::min(zc64, 16));
+        if (h->wht) {
+            // each z chunk recomputes one element layer, and a grid just
+            // above one resident round of blocks doubles the time: chunk
+            // length from the occupancy of the FP32 post-sweep and the FP64
+            // apply (the heaviest launches of each precision)
+            const int64_t tp = (int64_t)h->wht_ntx * h->wht_nty;
+            int dev = 0, ncu = 0, o32 = 0, o64 = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &o32, h->wht_pf32 ? reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, true>)
+                                  : reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, false>),
+                TPB, 0);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &o64, h->wht_pf64 ? reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, true>)
+                                  : reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, false>),
+                TPB, 0);
+            ncu = std::max(ncu, 1);
+            h->zchunk = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o32, 1));
+            h->zchunk64 = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o64, 1));
+        }
         if (const char *ev = getenv("TMG_ZCHUNK")) h->zchunk = std::max(1, std::min(atoi(ev), 64));
         if (const char *ev = getenv("TMG_ZCHUNK64")) h->zchunk64 = std::max(1, std::min(atoi(ev), 64));
     }
@@ -1083,6 +1195,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     }
     if (hipMemcpyAsync(h->K0_64, kop.data(), sizeof(double) * kop.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->K0_32, ke32.data(), sizeof(float) * ke32.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->kw64_d, h->kw64.g, sizeof(h->kw64.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->kw32_d, h->kw32.g, sizeof(h->kw32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
