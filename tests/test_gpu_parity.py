@@ -368,3 +368,30 @@ def test_multiresolution_matches_oracle(oracle_mod, precision):
     W, w = oracle_mod.filter_matrix(2 * nelx, 2 * nely, 2 * nelz, 1.5, 2, el=0.5, eh=0.5, ew=0.5)
     g0 = o.grad0_mr(2, ufull, xd)
     assert _rel(s.grad(ufull, xd, filt=f), W.T @ (g0 / w)) < 1e-12
+
+
+@pytest.mark.parametrize("grid", [(8, 4, 4), (20, 6, 10)])
+def test_multiresolution_spectral_matches_stencil(monkeypatch, grid):
+    """The matrix-free multiresolution apply (spectral element form with the
+    sub-element moduli's transform) against the assembled fine-level block
+    stencil (TMG_MR_STENCIL=1) on the same densities: FP64 apply and sweeps to
+    rounding, and the mixed-precision V-cycle to FP32 rounding."""
+    import tmg
+    nelx, nely, nelz = grid
+    _say(f"multiresolution spectral vs stencil {grid}")
+    prob = P.cantilever(nelx, nely, nelz)
+    xd = P.random_density(2 * nelx, 2 * nely, 2 * nelz, 41)
+    v = np.random.default_rng(41).standard_normal(prob.nfree)
+    b = np.random.default_rng(42).standard_normal(prob.nfree)
+    out = {}
+    for tag, env in (("wht", "0"), ("stencil", "1")):
+        monkeypatch.setenv("TMG_MR_STENCIL", env)
+        for prec in ("fp64", "mixed"):
+            s = tmg.Solver(prob, 2, precision=prec, mr=2)
+            s.update_density(xd)
+            out[tag, prec] = (s.apply(1, v), s.smooth(1, b, v, 2), s.vcycle(b))
+            s.close()
+    for i, tol in enumerate((1e-13, 1e-12, 1e-10)):
+        assert _rel(out["wht", "fp64"][i], out["stencil", "fp64"][i]) < tol, i
+    assert _rel(out["wht", "mixed"][2], out["stencil", "mixed"][2]) < 1e-5
+    assert _rel(out["wht", "mixed"][2], out["wht", "fp64"][2]) < 1e-5

@@ -588,9 +588,42 @@ __host__ __device__ constexpr int wht_slot(int c, int d, int k) {
     for (int kk = 0; kk < k; ++kk) s += (WHT_NZ[c][d] >> kk) & 1;
     return s;
 }
+// Multiresolution (2x2x2 density sub-elements, element matrix sum_s E_s K_s):
+// each sub-element integrand is the reflection of sub-element 0's, so in the
+// same basis the coupling becomes
+//     (H y_c)(k) = sum_{d,l} A[c][d][k][l] (H E)(k ^ l ^ e_c ^ e_d) (H x_d)(l)
+// with H E the transform of the 8 sub-element moduli (s = sy + 2sx + 4sz) and
+// a fixed table A. Its nonzero pattern, identical for every box: row c*8+k,
+// bit 8d+l (279 entries; the rows and columns of the translation mode k = 0
+// are empty). The host derives A from the integrands and checks both the
+// structure and this pattern.
+constexpr uint32_t WHT_MR_NZ[24] = {0x000000, 0x00CCAA, 0xF0AACC, 0xF0EEEE, 0xCC00F0, 0xCCCCFA, 0xFCAAFC, 0xFCEEFE,
+                                    0x000000, 0xF0AACC, 0x00CCAA, 0xF0EEEE, 0xAAF000, 0xFAFACC, 0xAAFCAA, 0xFAFEEE,
+                                    0x000000, 0xAAF000, 0xCC00F0, 0xEEF0F0, 0xF0AACC, 0xFAFACC, 0xFCAAFC, 0xFEFAFC};
+constexpr int WHT_MR_NCOEF = 279;
+// slot of coefficient (row 8c+k, column 8d+l) in the table, -1 where zero;
+// built at compile time so that unrolled kernels index it with constants
+struct WhtMrSlots {
+    short s[24][24];
+};
+constexpr WhtMrSlots wht_mr_slots() {
+    WhtMrSlots t{};
+    int n = 0;
+    for (int r = 0; r < 24; ++r)
+        for (int b = 0; b < 24; ++b) {
+            const bool nz = (WHT_MR_NZ[r] >> b) & 1;
+            t.s[r][b] = (short)(nz ? n : -1);
+            n += nz;
+        }
+    return t;
+}
+constexpr WhtMrSlots WHT_MR_SLOT = wht_mr_slots();
+static_assert(WHT_MR_SLOT.s[23][23] == WHT_MR_NCOEF - 1, "279 nonzero coefficients");
+__host__ __device__ constexpr int wht_mr_slot(int c, int k, int d, int l) { return WHT_MR_SLOT.s[8 * c + k][8 * d + l]; }
+// host-side coefficient tables (single resolution uses the first WHT_NCOEF)
 template <typename T>
 struct KWht {
-    T g[WHT_NCOEF];
+    T g[WHT_MR_NCOEF];
 };
 // Smallest z chunk of the spectral apply whose grid fits in one resident
 // round (cap blocks), else the chunk minimising rounds x (layers per block).
@@ -617,9 +650,10 @@ __device__ inline void wht4(const T (&v)[4], T (&P)[4]) {
 }
 
 // tyl: log2 of the tile's y extent TY (TX = TPB / TY); ntx: tiles in x;
-// kw_: the WHT_NCOEF coefficients G in slot order. PF: the loads of the next
-// node plane are issued before the current layer's arithmetic.
-template <typename T, typename TS, int SRC, int EPI, bool PF>
+// kw_: the coefficients in slot order (G, or A for NSUB = 8). PF: the loads of
+// the next node plane are issued before the current layer's arithmetic.
+// NSUB = 8: multiresolution, moduli E[8e + s].
+template <typename T, typename TS, int SRC, int EPI, bool PF, int NSUB = 1>
 __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_, int zchunk_, int tyl_, int ntx_,
                                                  const T *__restrict__ x_, const T *__restrict__ E_,
                                                  const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
@@ -687,10 +721,14 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
             w[c] = nv[0][c];
         }
     };
-    auto modulus = [&](int ez) {
+    auto modulus = [&](int ez, T (&dst)[NSUB]) {
         const bool ok = ecol && ez >= 0 && ez < nelz;
-        const T e = E_[(int64_t)min(max(ez, 0), nelz - 1) * nelx * nely + ecol_idx];
-        return ok ? e : T(0);
+        const int64_t e = (int64_t)min(max(ez, 0), nelz - 1) * nelx * nely + ecol_idx;
+#pragma unroll
+        for (int q = 0; q < NSUB; ++q) {
+            const T v = E_[NSUB * e + q];
+            dst[q] = ok ? v : T(0);
+        }
     };
     // coefficients through the constant address space (scalar loads); the
     // base is re-materialised per component so that only a third of them
@@ -701,7 +739,8 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
     load_raw(z0 - 1);
     transform(Pt, wt);
     if (PF) load_raw(z0);
-    T Epre = modulus(z0 - 1);
+    T Epre[NSUB], Ee[NSUB];
+    modulus(z0 - 1, Epre);
     T Qtp[3][4];                     // top-plane spectra of the previous layer
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -718,8 +757,9 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
         if (!PF) load_raw(L + 1);
         transform(Pt, wt);               // plane L + 1
         if (PF) load_raw(L + 2);         // in flight during this layer's arithmetic
-        const T Ee = Epre;
-        Epre = modulus(L + 1);
+#pragma unroll
+        for (int q = 0; q < NSUB; ++q) Ee[q] = Epre[q];
+        modulus(L + 1, Epre);
         // element spectrum x^(k), k = kxy + 4 kz (x^(0) is never used)
         T xh[3][8];
 #pragma unroll
@@ -731,18 +771,61 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
             }
         // y^ = E G x^ (y^(0) = 0: translations are in the null space)
         T yh[3][8];
+        if constexpr (NSUB == 1) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c > 0) asm volatile("" : "+s"(kp) : "v"(yh[c - 1][7]));
-            yh[c][0] = T(0);
+            for (int c = 0; c < 3; ++c) {
+                if (c > 0) asm volatile("" : "+s"(kp) : "v"(yh[c - 1][7]));
+                yh[c][0] = T(0);
 #pragma unroll
-            for (int k = 1; k < 8; ++k) {
-                T acc = T(0);
+                for (int k = 1; k < 8; ++k) {
+                    T acc = T(0);
 #pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    if ((WHT_NZ[c][d] >> k) & 1)
-                        acc = fma(kp[wht_slot(c, d, k)], xh[d][k ^ WHT_BIT[c] ^ WHT_BIT[d]], acc);
-                yh[c][k] = Ee * acc;
+                    for (int d = 0; d < 3; ++d)
+                        if ((WHT_NZ[c][d] >> k) & 1)
+                            acc = fma(kp[wht_slot(c, d, k)], xh[d][k ^ WHT_BIT[c] ^ WHT_BIT[d]], acc);
+                    yh[c][k] = Ee[0] * acc;
+                }
+            }
+        } else {
+            // transform of the sub-element moduli, then each spectral row as
+            // sum over modes m of (H E)(m) times its group of coefficients
+            T Eh[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Eh[q] = Ee[q];
+#pragma unroll
+            for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (!(q & bit)) {
+                        const T u = Eh[q], v = Eh[q | bit];
+                        Eh[q] = u + v;
+                        Eh[q | bit] = u - v;
+                    }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                yh[c][0] = T(0);
+#pragma unroll
+                for (int k = 1; k < 8; ++k) {
+                    // only one row's coefficients occupy SGPRs at a time
+                    asm volatile("" : "+s"(kp) : "v"(k > 1 ? yh[c][k - 1] : (c > 0 ? yh[c - 1][7] : Eh[0])));
+                    T grp[8];
+                    bool has[8] = {false, false, false, false, false, false, false, false};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int l = 1; l < 8; ++l)
+                            if ((WHT_MR_NZ[8 * c + k] >> (8 * d + l)) & 1) {
+                                const int m = k ^ l ^ WHT_BIT[c] ^ WHT_BIT[d];
+                                const T coef = kp[wht_mr_slot(c, k, d, l)];
+                                grp[m] = has[m] ? fma(coef, xh[d][l], grp[m]) : coef * xh[d][l];
+                                has[m] = true;
+                            }
+                    T acc = T(0);
+#pragma unroll
+                    for (int m = 0; m < 8; ++m)
+                        if (has[m]) acc = fma(Eh[m], grp[m], acc);
+                    yh[c][k] = acc;
+                }
             }
         }
         asm volatile("" : "+s"(kp) : "v"(yh[2][7]));

@@ -233,14 +233,22 @@ template <> const float *KWof<float>(tmg_solver *h) { return h->kw32_d; }
 template <typename T, typename TS, int SRC, int EPI>
 void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
     const DevLevel &lv = h->lev[0];
-    if (h->wht && h->nsub == 1 && a.fin < 0) {
+    if (h->wht && a.fin < 0 && (h->nsub == 1 || SRC == SRC_X)) {
         dim3 gw((unsigned)(h->wht_ntx * h->wht_nty), (unsigned)((lv.nz + a.zchunk - 1) / a.zchunk));
 #define WHT_ARGS a.nelx, a.nely, a.nelz, a.zchunk, h->wht_tyl, h->wht_ntx, a.x, a.E, a.mask, a.b, a.invD, a.r, a.ec, \
                  a.cnx, a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KWof<T>(h), h->xcd_wht
-        if (std::is_same<T, double>::value ? h->wht_pf64 : h->wht_pf32)
+        const bool pf = std::is_same<T, double>::value ? h->wht_pf64 : h->wht_pf32;
+        if (h->nsub == 8) {
+            // multiresolution: one form per precision (FP64 prefetching)
+            if constexpr (SRC == SRC_X) {
+                constexpr bool PF8 = std::is_same<T, double>::value;
+                hipLaunchKernelGGL((k_ebe_wht<T, TS, SRC, EPI, PF8, 8>), gw, dim3(TPB), 0, h->st, WHT_ARGS);
+            }
+        } else if (pf) {
             hipLaunchKernelGGL((k_ebe_wht<T, TS, SRC, EPI, true>), gw, dim3(TPB), 0, h->st, WHT_ARGS);
-        else
+        } else {
             hipLaunchKernelGGL((k_ebe_wht<T, TS, SRC, EPI, false>), gw, dim3(TPB), 0, h->st, WHT_ARGS);
+        }
 #undef WHT_ARGS
         return;
     }
@@ -960,9 +968,10 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->mr = mr_n;
     h->nsub = mr_n * mr_n * mr_n;
     h->pprec = precision;
-    // eight weighted integrands per element make the matrix-free apply 8x
-    // the work; the assembled stencil is read at memory speed instead
-    h->fine_stencil = h->nsub > 1 && !getenv_flag("TMG_MR_MATRIX_FREE");
+    // multiresolution: matrix-free in spectral element form (a tenth of the
+    // memory traffic of the assembled fine stencil); TMG_MR_STENCIL=1
+    // assembles the fine level into a block stencil instead
+    h->fine_stencil = h->nsub > 1 && getenv_flag("TMG_MR_STENCIL");
     if (h->nsub > 1) h->fused = false;
     if (const char *ev = getenv("TMG_XCD_REMAP")) h->xcd_remap = ev[0] == '1';
     h->xcd_ebe = getenv_flag("TMG_XCD_EBE") ? 1 : 0;
@@ -1081,7 +1090,50 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     // transform of g / 8; accepted only if Ke has the reflection structure
     // (every entry reproduced) and G vanishes outside the WHT_NZ pattern
     h->wht = false;
-    if (h->nsub == 1 && !getenv_flag("TMG_EBE_GENERIC") && !(getenv("TMG_EBE_WHT") && getenv("TMG_EBE_WHT")[0] == '0')) {
+    const bool wht_allowed = !getenv_flag("TMG_EBE_GENERIC") && !(getenv("TMG_EBE_WHT") && getenv("TMG_EBE_WHT")[0] == '0');
+    if (h->nsub == 8 && !h->fine_stencil && wht_allowed) {
+        // A[c][d][k][l] from the sub-element couplings
+        //   C_s[c][d][k][l] = 1/64 sum_ab H[k][a] K_s[3a+c][3b+d] H[l][b],
+        // which must equal A (H)[m][s] with m = k ^ l ^ e_c ^ e_d
+        const int bitk[3] = {2, 1, 4};
+        auto Hm = [](int k, int a) { return (__builtin_popcount(k & a) & 1) ? -1.0 : 1.0; };
+        std::vector<double> Cs((size_t)8 * 576);
+        double cmax = 0.0;
+        for (int sb = 0; sb < 8; ++sb)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d)
+                    for (int k = 0; k < 8; ++k)
+                        for (int l = 0; l < 8; ++l) {
+                            double sum = 0.0;
+                            for (int a = 0; a < 8; ++a)
+                                for (int b = 0; b < 8; ++b)
+                                    sum += Hm(k, a) * kop[576 * sb + (3 * a + c) * 24 + 3 * b + d] * Hm(l, b);
+                            const double v = sum / 64.0;
+                            Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] = v;
+                            cmax = std::max(cmax, std::fabs(v));
+                        }
+        bool ok = true;
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 8; ++k)
+                    for (int l = 0; l < 8; ++l) {
+                        const int m = k ^ l ^ bitk[c] ^ bitk[d];
+                        double al = 0.0;
+                        for (int sb = 0; sb < 8; ++sb) al += Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] * Hm(m, sb);
+                        al /= 8.0;
+                        for (int sb = 0; sb < 8; ++sb)
+                            ok = ok && std::fabs(Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] - al * Hm(m, sb)) <=
+                                           1e-13 * cmax;
+                        if ((WHT_MR_NZ[8 * c + k] >> (8 * d + l)) & 1) {
+                            h->kw64.g[wht_mr_slot(c, k, d, l)] = al;
+                            h->kw32.g[wht_mr_slot(c, k, d, l)] = (float)al;
+                        } else {
+                            ok = ok && std::fabs(al) <= 1e-13 * cmax;
+                        }
+                    }
+        h->wht = ok;
+    }
+    if (h->nsub == 1 && wht_allowed) {
         const int bitc[3] = {1, 0, 2};
         auto sgn = [&](int a, int c) { return ((a >> bitc[c]) & 1) ? -1.0 : 1.0; };
         double kmax = 0.0;
@@ -1118,8 +1170,10 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                         ok = ok && std::fabs(G[c][d][k]) <= 1e-13 * gmax;
                     }
                 }
-        if (ok) {
-            h->wht = true;
+        h->wht = ok;
+    }
+    if (h->wht) {
+        {
             if (const char *ev = getenv("TMG_WHT_PREFETCH")) h->wht_pf32 = h->wht_pf64 = ev[0] != '0';
             if (const char *ev = getenv("TMG_XCD_WHT")) h->xcd_wht = ev[0] != '0';
             // tile shape: TY in {8, 16, 32} with the fewest tiles per node plane
@@ -1143,7 +1197,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
     if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
-        (rc = dalloc(h, &h->kw64_d, WHT_NCOEF)) || (rc = dalloc(h, &h->kw32_d, WHT_NCOEF)) ||
+        (rc = dalloc(h, &h->kw64_d, WHT_MR_NCOEF)) || (rc = dalloc(h, &h->kw32_d, WHT_MR_NCOEF)) ||
         (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
@@ -1170,14 +1224,18 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
             int dev = 0, ncu = 0, o32 = 0, o64 = 0;
             (void)hipGetDevice(&dev);
             (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &o32, h->wht_pf32 ? reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, true>)
-                                  : reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, false>),
-                TPB, 0);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &o64, h->wht_pf64 ? reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, true>)
-                                  : reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, false>),
-                TPB, 0);
+            const void *k32, *k64;
+            if (h->nsub == 8) {
+                k32 = reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, false, 8>);
+                k64 = reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, true, 8>);
+            } else {
+                k32 = h->wht_pf32 ? reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, true>)
+                                  : reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_POSTR, false>);
+                k64 = h->wht_pf64 ? reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, true>)
+                                  : reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, false>);
+            }
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o32, k32, TPB, 0);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o64, k64, TPB, 0);
             ncu = std::max(ncu, 1);
             h->zchunk = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o32, 1));
             h->zchunk64 = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o64, 1));
