@@ -259,6 +259,41 @@ def test_spectral_apply_matches_dense_kernel(monkeypatch, grid):
     assert _rel(out["wht32"], out["wht64"]) < 1e-5
 
 
+@pytest.mark.parametrize("support_x", [0, 1])
+def test_level1_matrix_free_matches_stencil(oracle_mod, monkeypatch, support_x):
+    """Level 1 (the first Galerkin level) applied matrix-free in the
+    multiresolution form (8 children's moduli, Galerkin integrands' table)
+    against the assembled stencil (TMG_L1_STENCIL=1) and the oracle. A support
+    on the odd plane x = 1 has fixed fine DOFs interpolating from free coarse
+    DOFs: the operator is then not a sum over coarse elements and level 1 keeps
+    the stencil; both cases must match the oracle."""
+    nelx, nely, nelz, L = 16, 8, 8, 3
+    _say(f"level-1 matrix-free, support x = {support_x}")
+    prob = P.from_rows(nelx, nely, nelz, [[support_x, support_x, 0, nely, 0, nelz, 1, 1, 1]],
+                       [[nelx, nelx, 0, 0, 0, nelz, 0, -1, 0]])
+    x = P.random_density(nelx, nely, nelz, 51)
+    o = _oracle(oracle_mod, prob, x, L)
+    rng = np.random.default_rng(51)
+    v1 = rng.standard_normal(o.nfree(1))
+    b1 = rng.standard_normal(o.nfree(1))
+    bf = rng.standard_normal(prob.nfree)
+    out = {}
+    for tag, env in (("free", "0"), ("stencil", "1")):
+        monkeypatch.setenv("TMG_L1_STENCIL", env)
+        s = _setup(prob, x, L)
+        out[tag] = (s.apply(2, v1), s.smooth(2, b1, v1, 2), s.vcycle(bf))
+        s.close()
+        s = _setup(prob, x, L, "mixed")
+        out[tag + "32"] = s.vcycle(bf)
+        s.close()
+    assert _rel(out["free"][0], o.spmv(1, v1)) < 1e-13
+    assert _rel(out["free"][1], o.smooth(1, b1, v1, 2)) < 1e-12
+    for i, tol in enumerate((1e-13, 1e-12, 1e-10)):
+        assert _rel(out["free"][i], out["stencil"][i]) < tol, i
+    assert _rel(out["free32"], out["stencil32"]) < 1e-5
+    assert _rel(out["free32"], out["free"][2]) < 1e-5
+
+
 def test_bitwise_repeatable():
     _say("repeatable")
     prob = P.cantilever(16, 8, 8)

@@ -1328,6 +1328,25 @@ __global__ __launch_bounds__(TPB) void k_modulus(int64_t ne, const double *__res
 // Multiresolution moduli: E[8e + s] of sub-element s = sy + 2 sx + 4 sz of
 // element e, from densities x on the density mesh (2nelx x 2nely x 2nelz,
 // y fastest) (Implementation/GlobalStiffMatrix.m:19-62).
+// Moduli of the 8 fine elements of each coarse element, in child order
+// s = sy + 2sx + 4sz: E1[8C + s] = E0[child s of C] (the level-1 Galerkin
+// operator applied in the multiresolution form).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_children_moduli(int cnelx, int cnely, int cnelz, const T *__restrict__ E0,
+                                                         T *__restrict__ E1) {
+    const int64_t q = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t nc = (int64_t)cnelx * cnely * cnelz;
+    if (q >= 8 * nc) return;
+    const int64_t C = q >> 3;
+    const int sidx = (int)(q & 7);
+    const int cz = (int)(C / ((int64_t)cnelx * cnely));
+    const int64_t r = C - (int64_t)cz * cnelx * cnely;
+    const int cx = (int)(r / cnely), cy = (int)(r - (int64_t)cx * cnely);
+    const int fx = 2 * cx + ((sidx >> 1) & 1), fy = 2 * cy + (sidx & 1), fz = 2 * cz + (sidx >> 2);
+    const int fnelx = 2 * cnelx, fnely = 2 * cnely;
+    E1[q] = E0[(int64_t)fz * fnelx * fnely + (int64_t)fx * fnely + fy];
+}
+
 __global__ __launch_bounds__(TPB) void k_modulus_mr(int nelx, int nely, int nelz, const double *__restrict__ x,
                                                     double E0, double emin, double p, double *__restrict__ E64,
                                                     float *__restrict__ E32) {

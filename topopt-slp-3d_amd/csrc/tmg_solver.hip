@@ -61,6 +61,15 @@ struct tmg_solver {
     float *K0_32 = nullptr;
     double *E64 = nullptr;
     float *E32 = nullptr;
+    // level 1 applied matrix-free (the Galerkin operator of an 8-child coarse
+    // element has the multiresolution form): moduli of the children, table
+    bool l1_wht = false;
+    bool l1_maskok = false;           // every fixed fine DOF interpolates only from fixed coarse DOFs
+    double *E1_64 = nullptr;
+    float *E1_32 = nullptr;
+    double *kwg64_d = nullptr;
+    float *kwg32_d = nullptr;
+    int l1_tyl = 4, l1_ntx = 1, l1_nty = 1, l1_zc32 = 4, l1_zc64 = 4;
     double *xdens = nullptr;
     double *Ahat[2] = {nullptr, nullptr};
     // coarsest grid
@@ -110,6 +119,8 @@ struct tmg_solver {
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
     KWht<float> kw32{};
+    KWht<double> kwg64{};             // level-1 (Galerkin) table
+    KWht<float> kwg32{};
     double *kw64_d = nullptr;         // the same coefficients on the device
     float *kw32_d = nullptr;
     int wht_tyl = 4, wht_ntx = 1, wht_nty = 1;
@@ -201,8 +212,100 @@ template <> const float *invof<float>(const DevLevel &l) { return l.inv32; }
 // Grid of the transfer kernels: in-plane blocks x planes.
 inline dim3 plane_grid(int nx, int ny, int nz) { return dim3((unsigned)nblk((int64_t)nx * ny), (unsigned)nz); }
 
+// Coupling table A of the multiresolution form from 8 sub-element matrices K
+// (8 x 576, row-major 24x24 each): C_s[c][d][k][l] = 1/64 sum_ab H[k][a]
+// K_s[3a+c][3b+d] H[l][b] must equal A[c][d][k][l] H[m][s] with
+// m = k ^ l ^ e_c ^ e_d, and A must vanish outside WHT_MR_NZ.
+bool wht_mr_table(const double *K, KWht<double> &t64, KWht<float> &t32) {
+    const int bitk[3] = {2, 1, 4};
+    auto Hm = [](int k, int a) { return (__builtin_popcount(k & a) & 1) ? -1.0 : 1.0; };
+    std::vector<double> Cs((size_t)8 * 576);
+    double cmax = 0.0;
+    for (int sb = 0; sb < 8; ++sb)
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 8; ++k)
+                    for (int l = 0; l < 8; ++l) {
+                        double sum = 0.0;
+                        for (int a = 0; a < 8; ++a)
+                            for (int b = 0; b < 8; ++b)
+                                sum += Hm(k, a) * K[576 * sb + (3 * a + c) * 24 + 3 * b + d] * Hm(l, b);
+                        const double v = sum / 64.0;
+                        Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] = v;
+                        cmax = std::max(cmax, std::fabs(v));
+                    }
+    bool ok = true;
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < 8; ++k)
+                for (int l = 0; l < 8; ++l) {
+                    const int m = k ^ l ^ bitk[c] ^ bitk[d];
+                    double al = 0.0;
+                    for (int sb = 0; sb < 8; ++sb) al += Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] * Hm(m, sb);
+                    al /= 8.0;
+                    for (int sb = 0; sb < 8; ++sb)
+                        ok = ok && std::fabs(Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] - al * Hm(m, sb)) <=
+                                       1e-13 * cmax;
+                    if ((WHT_MR_NZ[8 * c + k] >> (8 * d + l)) & 1) {
+                        t64.g[wht_mr_slot(c, k, d, l)] = al;
+                        t32.g[wht_mr_slot(c, k, d, l)] = (float)al;
+                    } else {
+                        ok = ok && std::fabs(al) <= 1e-13 * cmax;
+                    }
+                }
+    return ok;
+}
+
+// Galerkin integrands of the 8 fine elements of a coarse element: P_s' Ke P_s
+// with P_s the trilinear interpolation (Implementation/Prolongation.m weights)
+// from the coarse element's 8 nodes to fine element s's 8 nodes.
+void galerkin_integrands(const double *Ke, double *K8) {
+    for (int sb = 0; sb < 8; ++sb) {
+        const int sy = sb & 1, sx = (sb >> 1) & 1, sz = sb >> 2;
+        double P[24 * 24] = {0.0};
+        for (int a = 0; a < 8; ++a) {
+            const double pos[3] = {0.5 * (sx + ((a >> 1) & 1)), 0.5 * (sy + (a & 1)), 0.5 * (sz + (a >> 2))};
+            for (int b = 0; b < 8; ++b) {
+                const double bc[3] = {(double)((b >> 1) & 1), (double)(b & 1), (double)(b >> 2)};
+                double w = 1.0;
+                for (int ax = 0; ax < 3; ++ax) w *= 1.0 - std::fabs(pos[ax] - bc[ax]);
+                for (int c = 0; c < 3; ++c) P[(3 * a + c) * 24 + 3 * b + c] = w;
+            }
+        }
+        double KP[24 * 24];
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double sum = 0.0;
+                for (int q = 0; q < 24; ++q) sum += Ke[i * 24 + q] * P[q * 24 + j];
+                KP[i * 24 + j] = sum;
+            }
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double sum = 0.0;
+                for (int q = 0; q < 24; ++q) sum += P[q * 24 + i] * KP[q * 24 + j];
+                K8[576 * sb + i * 24 + j] = sum;
+            }
+    }
+}
+
+// tile shape (TY in {8, 16, 32}, the fewest tiles per node plane) of the
+// spectral apply on an nx x ny node plane
+void wht_tiles(int nx, int ny, int &tyl, int &ntx, int &nty) {
+    int64_t best = -1;
+    for (int t = 3; t <= 5; ++t) {
+        const int TY = 1 << t, TX = TPB / TY;
+        const int ax = (nx + TX - 2) / (TX - 1), ay = (ny + TY - 2) / (TY - 1);
+        const int64_t nt = (int64_t)ax * ay;
+        if (best < 0 || nt < best || (nt == best && t == 4)) { best = nt; tyl = t; ntx = ax; nty = ay; }
+    }
+}
+
 int64_t apply_blocks(tmg_solver *h, int l, bool dbl) {
     const DevLevel &lv = h->lev[l];
+    if (l == 1 && h->l1_wht) {
+        const int zc = dbl ? h->l1_zc64 : h->l1_zc32;
+        return (int64_t)h->l1_ntx * h->l1_nty * ((lv.nz + zc - 1) / zc);
+    }
     if (l == 0 && !h->fine_stencil) {
         int64_t pb = h->wht ? (int64_t)h->wht_ntx * h->wht_nty : nblk((int64_t)lv.nx * lv.ny);
         const int zc = dbl ? h->zchunk64 : h->zchunk;
@@ -287,9 +390,30 @@ StArgs<T> st_args(tmg_solver *h, int l) {
     return a;
 }
 
+template <typename T>
+const T *E1of(tmg_solver *h);
+template <> const double *E1of<double>(tmg_solver *h) { return h->E1_64; }
+template <> const float *E1of<float>(tmg_solver *h) { return h->E1_32; }
+template <typename T>
+const T *KWGof(tmg_solver *h);
+template <> const double *KWGof<double>(tmg_solver *h) { return h->kwg64_d; }
+template <> const float *KWGof<float>(tmg_solver *h) { return h->kwg32_d; }
+
 template <typename T, int EPI>
 void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial, const int *done) {
     const DevLevel &lv = h->lev[l];
+    if (l == 1 && h->l1_wht) {
+        // level-1 Galerkin operator, matrix-free: the 8 children's moduli
+        // times the Galerkin integrands' table (multiresolution form)
+        const int zc = std::is_same<T, double>::value ? h->l1_zc64 : h->l1_zc32;
+        constexpr bool PF8 = std::is_same<T, double>::value;
+        dim3 gw((unsigned)(h->l1_ntx * h->l1_nty), (unsigned)((lv.nz + zc - 1) / zc));
+        hipLaunchKernelGGL((k_ebe_wht<T, T, SRC_X, EPI, PF8, 8>), gw, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, zc,
+                           h->l1_tyl, h->l1_ntx, x, E1of<T>(h), lv.mask, b, invof<T>(lv), (const double *)nullptr,
+                           (const T *)nullptr, 1, 1, (const T *)nullptr, (const double *)nullptr, (double *)nullptr,
+                           (PcgScalars *)nullptr, out, partial, done, KWGof<T>(h), h->xcd_wht);
+        return;
+    }
     if (l == 0 && !h->fine_stencil) {
         EbeArgs<T, T> a = ebe_args<T, T>(h);
         a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
@@ -747,7 +871,8 @@ void destroy_all(tmg_solver *h) {
         dfree(h, lv.mask); dfree(h, lv.S64); dfree(h, lv.S32); dfree(h, lv.inv64); dfree(h, lv.inv32);
         dfree(h, lv.b); dfree(h, lv.x); dfree(h, lv.t); dfree(h, lv.d_free); dfree(h, lv.d_cidx);
     }
-    dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
+    dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d);
+    dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
     dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
@@ -826,6 +951,14 @@ int refresh(tmg_solver *h) {
     else
         hipLaunchKernelGGL(k_modulus_mr, dim3(nblk(l0.ne * 8)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->xdens,
                            h->E0, h->emin, h->penal, h->E64, h->prec == TMG_MIXED ? h->E32 : nullptr);
+    if (h->l1_wht) {
+        const DevLevel &l1 = h->lev[1];
+        hipLaunchKernelGGL((k_children_moduli<double>), dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l1.nelx, l1.nely, l1.nelz,
+                           h->E64, h->E1_64);
+        if (h->prec == TMG_MIXED)
+            hipLaunchKernelGGL((k_children_moduli<float>), dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l1.nelx, l1.nely,
+                               l1.nelz, h->E32, h->E1_32);
+    }
     hipLaunchKernelGGL(k_diag_fine, dim3(nblk(l0.nn)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, h->nsub, h->E64,
                        h->K0_64, l0.mask, om, l0.inv64, h->prec == TMG_MIXED ? l0.inv32 : nullptr);
     if (h->fine_stencil)
@@ -1036,6 +1169,26 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                         const int64_t fi = (int64_t)(2 * iz) * lf.nx * lf.ny + (int64_t)(2 * ix) * lf.ny + 2 * iy;
                         for (int c = 0; c < 3; ++c) cm[3 * I + c] = mask[3 * fi + c];
                     }
+            if (l == 1) {
+                // every fixed fine DOF may interpolate only from fixed coarse
+                // DOFs: then M_f P M_c = P M_c and the level-1 operator is
+                // M_c P' A P M_c, a sum over coarse elements
+                bool ok = true;
+                for (int fz = 0; ok && fz < lf.nz; ++fz)
+                    for (int fx = 0; ok && fx < lf.nx; ++fx)
+                        for (int fy = 0; ok && fy < lf.ny; ++fy) {
+                            const int64_t fi = (int64_t)fz * lf.nx * lf.ny + (int64_t)fx * lf.ny + fy;
+                            for (int c = 0; c < 3; ++c) {
+                                if (mask[3 * fi + c]) continue;
+                                for (int t = 0; t < 8; ++t) {
+                                    const int X = (t & 2) ? (fx + 1) / 2 : fx / 2, Y = (t & 1) ? (fy + 1) / 2 : fy / 2;
+                                    const int Z = (t & 4) ? (fz + 1) / 2 : fz / 2;
+                                    ok = ok && !cm[3 * ((int64_t)Z * cnx * cny + (int64_t)X * cny + Y) + c];
+                                }
+                            }
+                        }
+                h->l1_maskok = ok;
+            }
             mask.swap(cm);
         }
         int rc = l == 0 ? alloc_level(h, l, mask, prob->freedofs, prob->nfree) : alloc_level(h, l, mask);
@@ -1091,48 +1244,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     // (every entry reproduced) and G vanishes outside the WHT_NZ pattern
     h->wht = false;
     const bool wht_allowed = !getenv_flag("TMG_EBE_GENERIC") && !(getenv("TMG_EBE_WHT") && getenv("TMG_EBE_WHT")[0] == '0');
-    if (h->nsub == 8 && !h->fine_stencil && wht_allowed) {
-        // A[c][d][k][l] from the sub-element couplings
-        //   C_s[c][d][k][l] = 1/64 sum_ab H[k][a] K_s[3a+c][3b+d] H[l][b],
-        // which must equal A (H)[m][s] with m = k ^ l ^ e_c ^ e_d
-        const int bitk[3] = {2, 1, 4};
-        auto Hm = [](int k, int a) { return (__builtin_popcount(k & a) & 1) ? -1.0 : 1.0; };
-        std::vector<double> Cs((size_t)8 * 576);
-        double cmax = 0.0;
-        for (int sb = 0; sb < 8; ++sb)
-            for (int c = 0; c < 3; ++c)
-                for (int d = 0; d < 3; ++d)
-                    for (int k = 0; k < 8; ++k)
-                        for (int l = 0; l < 8; ++l) {
-                            double sum = 0.0;
-                            for (int a = 0; a < 8; ++a)
-                                for (int b = 0; b < 8; ++b)
-                                    sum += Hm(k, a) * kop[576 * sb + (3 * a + c) * 24 + 3 * b + d] * Hm(l, b);
-                            const double v = sum / 64.0;
-                            Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] = v;
-                            cmax = std::max(cmax, std::fabs(v));
-                        }
-        bool ok = true;
-        for (int c = 0; c < 3; ++c)
-            for (int d = 0; d < 3; ++d)
-                for (int k = 0; k < 8; ++k)
-                    for (int l = 0; l < 8; ++l) {
-                        const int m = k ^ l ^ bitk[c] ^ bitk[d];
-                        double al = 0.0;
-                        for (int sb = 0; sb < 8; ++sb) al += Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] * Hm(m, sb);
-                        al /= 8.0;
-                        for (int sb = 0; sb < 8; ++sb)
-                            ok = ok && std::fabs(Cs[(size_t)sb * 576 + ((c * 3 + d) * 8 + k) * 8 + l] - al * Hm(m, sb)) <=
-                                           1e-13 * cmax;
-                        if ((WHT_MR_NZ[8 * c + k] >> (8 * d + l)) & 1) {
-                            h->kw64.g[wht_mr_slot(c, k, d, l)] = al;
-                            h->kw32.g[wht_mr_slot(c, k, d, l)] = (float)al;
-                        } else {
-                            ok = ok && std::fabs(al) <= 1e-13 * cmax;
-                        }
-                    }
-        h->wht = ok;
-    }
+    if (h->nsub == 8 && !h->fine_stencil && wht_allowed) h->wht = wht_mr_table(kop.data(), h->kw64, h->kw32);
     if (h->nsub == 1 && wht_allowed) {
         const int bitc[3] = {1, 0, 2};
         auto sgn = [&](int a, int c) { return ((a >> bitc[c]) & 1) ? -1.0 : 1.0; };
@@ -1194,10 +1306,21 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
             }
         }
     }
+    // level 1 matrix-free: single resolution, spectral fine apply, masks that
+    // make P'AP element-decomposable (checked while the masks were built)
+    h->l1_wht = false;
+    if (h->wht && h->nsub == 1 && h->L >= 2 && h->l1_maskok && !getenv_flag("TMG_L1_STENCIL")) {
+        std::vector<double> K8(8 * 576);
+        galerkin_integrands(h->ke, K8.data());
+        h->l1_wht = wht_mr_table(K8.data(), h->kwg64, h->kwg32);
+        if (h->l1_wht) wht_tiles(h->lev[1].nx, h->lev[1].ny, h->l1_tyl, h->l1_ntx, h->l1_nty);
+    }
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
     if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
         (rc = dalloc(h, &h->kw64_d, WHT_MR_NCOEF)) || (rc = dalloc(h, &h->kw32_d, WHT_MR_NCOEF)) ||
+        (rc = dalloc(h, &h->kwg64_d, WHT_MR_NCOEF)) || (rc = dalloc(h, &h->kwg32_d, WHT_MR_NCOEF)) ||
+        (rc = dalloc(h, &h->E1_64, h->l1_wht ? l0.ne : 1)) || (rc = dalloc(h, &h->E1_32, h->l1_wht ? l0.ne : 1)) ||
         (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
@@ -1239,6 +1362,16 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
             ncu = std::max(ncu, 1);
             h->zchunk = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o32, 1));
             h->zchunk64 = wht_zchunk(tp, l0.nz, (int64_t)ncu * std::max(o64, 1));
+            if (h->l1_wht) {
+                int p32 = 0, p64 = 0;
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &p32, reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_JACOBI, false, 8>), TPB, 0);
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &p64, reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_JACOBI, true, 8>), TPB, 0);
+                const int64_t t1 = (int64_t)h->l1_ntx * h->l1_nty;
+                h->l1_zc32 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p32, 1));
+                h->l1_zc64 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p64, 1));
+            }
         }
         if (const char *ev = getenv("TMG_ZCHUNK")) h->zchunk = std::max(1, std::min(atoi(ev), 64));
         if (const char *ev = getenv("TMG_ZCHUNK64")) h->zchunk64 = std::max(1, std::min(atoi(ev), 64));
@@ -1255,6 +1388,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         hipMemcpyAsync(h->K0_32, ke32.data(), sizeof(float) * ke32.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kw64_d, h->kw64.g, sizeof(h->kw64.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kw32_d, h->kw32.g, sizeof(h->kw32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->kwg64_d, h->kwg64.g, sizeof(h->kwg64.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipMemcpyAsync(h->kwg32_d, h->kwg32.g, sizeof(h->kwg32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
