@@ -70,7 +70,9 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
         ex /= 2; ey /= 2; ez /= 2;
     }
     add(576 * 8); add(576 * 4);
+    add(WHT_MR_NCOEF * 8); add(WHT_MR_NCOEF * 4); add(WHT_MR_NCOEF * 8); add(WHT_MR_NCOEF * 4);   // spectral tables
     add(8 * ne0); add(4 * ne0); add(8 * ne0);
+    add(8 * ne0); add(4 * ne0);                                // level-1 children's moduli (upper bound)
     add(8 * ne1 * 576); add(8 * (L > 2 ? ne2 * 576 : 1));
     add(8 * ncoarse * ncoarse); add(8 * ncoarse); add(8 * ncoarse); add(4);
     for (int k = 0; k < 6; ++k) add(8 * ndof0);
