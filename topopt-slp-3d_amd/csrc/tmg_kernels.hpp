@@ -1040,6 +1040,61 @@ __global__ __launch_bounds__(TPB) void k_stencil(int nx_, int ny_, int nz_, cons
     if (EPI == EPI_DOT) block_reduce_store(dacc, a.partial, blk);
 }
 
+// Small stencil levels (too few nodes to fill the chip, so latency-bound):
+// one workgroup per 64 nodes and one wave per neighbour plane (dz = -1, 0, 1),
+// the three partial sums added through LDS in that order. Three times the
+// waves in flight, each with a third of the dependent loads.
+constexpr int ST3_THREADS = 192;
+template <typename T, int SRC, int EPI>
+__global__ __launch_bounds__(ST3_THREADS) void k_stencil3(int nx_, int ny_, int nz_, const T *__restrict__ x_,
+                                                          const T *__restrict__ S_, const uint8_t *__restrict__ mask_,
+                                                          const T *__restrict__ b_, const T *__restrict__ invD_,
+                                                          const T *__restrict__ ec_, int cnx_, int cny_,
+                                                          T *__restrict__ out_, double *__restrict__ partial_,
+                                                          const int *__restrict__ done_, int xcd_) {
+    static_assert(EPI != EPI_DOT, "the split form has no dot-product epilogue");
+    if (done_ && *done_) return;
+    StArgs<T> a;
+    a.nx = nx_; a.ny = ny_; a.nz = nz_; a.x = x_; a.S = S_; a.mask = mask_; a.b = b_; a.invD = invD_;
+    a.ec = ec_; a.cnx = cnx_; a.cny = cny_; a.out = out_; a.partial = partial_; a.done = done_;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t nn = plane * nz;
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;   // wave wv: neighbour plane dz = wv - 1
+    const int64_t blk = xcd_ ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const int64_t i0 = blk * 64 + lane;
+    const bool act = i0 < nn;
+    __shared__ T part[2][3][64];
+    T y[3] = {T(0), T(0), T(0)};
+    const int iz = (int)(i0 / plane), ix = (int)((i0 / ny) % nx), iy = (int)(i0 % ny);
+    if (act) {
+        if (wv == 0) stencil_plane<T, SRC, -1>(a, i0, ix, iy, iz, plane, y);
+        else if (wv == 1) stencil_plane<T, SRC, 0>(a, i0, ix, iy, iz, plane, y);
+        else stencil_plane<T, SRC, 1>(a, i0, ix, iy, iz, plane, y);
+    }
+    if (wv > 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) part[wv - 1][c][lane] = y[c];
+    }
+    __syncthreads();
+    if (wv != 0 || !act) return;
+    T w[3];
+    stencil_source<T, SRC>(a, ix, iy, iz, i0, w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T yc = (y[c] + part[0][c][lane]) + part[1][c][lane];
+        const int64_t dof = 3 * i0 + c;
+        const bool m = a.mask[dof] != 0;
+        if (EPI == EPI_PLAIN) {
+            a.out[dof] = m ? yc : T(0);
+        } else if (EPI == EPI_RESID) {
+            a.out[dof] = m ? (a.b[dof] - yc) : T(0);
+        } else {  // EPI_JACOBI
+            a.out[dof] = m ? (w[c] + a.invD[dof] * (a.b[dof] - yc)) : T(0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Transfers: trilinear prolongation P (vertex-centred, weights 1, 1/2, 1/4,
 // 1/8, Implementation/Prolongation.m:15-37) restricted to free DOFs, and

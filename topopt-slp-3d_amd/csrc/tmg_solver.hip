@@ -127,6 +127,7 @@ struct tmg_solver {
     bool fused = false;               // fused fine-level V-cycle / direction update (nu1 = nu2 = 1)
     bool fused_coarse = true;         // fused stencil-level sweeps (nu1 = nu2 = 1)
     int64_t fused_maxnodes = FUSED_LEVEL_MAXNODES;  // largest stencil level that uses them
+    int64_t st3_maxnodes = 200000;    // largest stencil level swept in the three-wave split form
     bool fine_r = true;               // fine-level cycle reading the FP64 residual, r's in its post-sweep
     // PCG reductions finished by the producing kernel's last block instead of
     // a k_finalize launch: measured much slower (each block's device-scope
@@ -390,6 +391,19 @@ StArgs<T> st_args(tmg_solver *h, int l) {
     return a;
 }
 
+// Stencil sweep of level l: the three-wave split form on small levels.
+template <typename T, int SRC, int EPI>
+void launch_stencil(tmg_solver *h, int l, const StArgs<T> &a) {
+    const DevLevel &lv = h->lev[l];
+    if (EPI != EPI_DOT && lv.nn <= h->st3_maxnodes) {
+        if constexpr (EPI != EPI_DOT)
+            hipLaunchKernelGGL((k_stencil3<T, SRC, EPI>), dim3((unsigned)((lv.nn + 63) / 64)), dim3(ST3_THREADS), 0,
+                               h->st, ST_ARGS(a));
+        return;
+    }
+    hipLaunchKernelGGL((k_stencil<T, SRC, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
+}
+
 template <typename T>
 const T *E1of(tmg_solver *h);
 template <> const double *E1of<double>(tmg_solver *h) { return h->E1_64; }
@@ -421,7 +435,7 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
     } else {
         StArgs<T> a = st_args<T>(h, l);
         a.x = x; a.b = b; a.out = out; a.partial = partial; a.done = done;
-        hipLaunchKernelGGL((k_stencil<T, SRC_X, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
+        launch_stencil<T, SRC_X, EPI>(h, l, a);
     }
 }
 
@@ -467,13 +481,13 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
         // stencil level, V(1,1): rh = b - A (M\b); then s = w + M\(b - A w), w = M\b + P eH
         StArgs<T> a = st_args<T>(h, l);
         a.b = b; a.out = xb; a.done = done;
-        hipLaunchKernelGGL((k_stencil<T, SRC_JZ, EPI_RESID>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
+        launch_stencil<T, SRC_JZ, EPI_RESID>(h, l, a);
         hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
                            lc.nz, xb, lc.mask, (T *)lc.b, done);
         T *ec = coarse_correction<T>(h, l, done);
         StArgs<T> c = st_args<T>(h, l);
         c.b = b; c.ec = ec; c.out = xa; c.done = done;
-        hipLaunchKernelGGL((k_stencil<T, SRC_POST, EPI_JACOBI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(c));
+        launch_stencil<T, SRC_POST, EPI_JACOBI>(h, l, c);
         return xa;
     }
     if (x0) {
@@ -1113,6 +1127,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_FINE_R")) h->fine_r = ev[0] != '0';
     if (const char *ev = getenv("TMG_FIN_FUSED")) h->fin_fused = ev[0] != '0';
     if (const char *ev = getenv("TMG_FUSED_MAXNODES")) h->fused_maxnodes = atoll(ev);
+    if (const char *ev = getenv("TMG_ST3_MAXNODES")) h->st3_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
