@@ -80,3 +80,19 @@ def test_server_rejects_bad_arguments_and_estimates():
     large = serve.estimate_bytes(P.cantilever(128, 64, 64), 5)
     assert 0 < small < large < 2e9
     assert 0 < serve.estimate_bytes(P.cantilever(128, 64, 64), 5, "fp64") < 2e9
+
+
+def test_serving_summary_counts_failed_requests():
+    """A request that fails (set-up or solve error) is counted and listed by
+    the serving summary; latency and throughput cover the completed ones."""
+    from tmg import serve
+    ok = [{"t_arrival": 0.1 * k, "t_done": 0.1 * k + 0.02, "nfree": 1000, "it": 11, "rc": 0, "relres": 5e-9, "i": k}
+          for k in range(9)]
+    bad = {"failed": True, "error": "tmg error -4: not positive definite", "i": 9, "nfree": 1000}
+    out = serve.summarize(ok + [bad])
+    assert out["requests"] == 10 and out["failed"] == 1
+    assert out["failed_requests"] == [(9, "tmg error -4: not positive definite")]
+    assert not out["all_converged"]
+    assert abs(out["p50_ms"] - 20.0) < 1e-9
+    out2 = serve.summarize(ok)
+    assert out2["failed"] == 0 and out2["all_converged"] and "failed_requests" not in out2

@@ -146,7 +146,11 @@ def run_stream(server, reqs, rate, seed=4, precision="mixed"):
     out = []
     for r, rid in zip(reqs, ids):
         ok, st = server.wait(rid)
-        res = server.result(rid)
+        try:
+            res = server.result(rid)
+        except TmgError as e:
+            # a request whose set-up or solve failed is reported, not fatal
+            res = {"failed": True, "error": str(e)}
         res.update(i=r["i"], nfree=r["prob"].nfree, grid=(r["prob"].nelx, r["prob"].nely, r["prob"].nelz))
         server.release(rid)
         out.append(res)
@@ -154,16 +158,24 @@ def run_stream(server, reqs, rate, seed=4, precision="mixed"):
 
 
 def summarize(records):
-    lat = np.array([r["t_done"] - r["t_arrival"] for r in records])
-    t_first = min(r["t_arrival"] for r in records)
-    t_last = max(r["t_done"] for r in records)
+    """Latency and throughput over the completed requests; failed ones are
+    counted and listed (request index and error)."""
+    failed = [r for r in records if r.get("failed")]
+    done = [r for r in records if not r.get("failed")]
+    lat = np.array([r["t_done"] - r["t_arrival"] for r in done])
+    t_first = min(r["t_arrival"] for r in done)
+    t_last = max(r["t_done"] for r in done)
     span = t_last - t_first
-    dofit = sum(r["nfree"] * (r["it"] - 1) for r in records)
-    return {"requests": len(records), "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "max_ms": float(lat.max() * 1e3),
-            "mean_ms": float(lat.mean() * 1e3), "makespan_s": span, "requests_per_s": len(records) / span,
-            "dof_iters_per_s": dofit / span,
-            "all_converged": all(r["rc"] == 0 and r["relres"] <= 1e-8 for r in records)}
+    dofit = sum(r["nfree"] * (r["it"] - 1) for r in done)
+    out = {"requests": len(records), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+           "p99_ms": float(np.percentile(lat, 99) * 1e3), "max_ms": float(lat.max() * 1e3),
+           "mean_ms": float(lat.mean() * 1e3), "makespan_s": span, "requests_per_s": len(done) / span,
+           "dof_iters_per_s": dofit / span,
+           "all_converged": not failed and all(r["rc"] == 0 and r["relres"] <= 1e-8 for r in done),
+           "failed": len(failed)}
+    if failed:
+        out["failed_requests"] = [(r["i"], r["error"]) for r in failed[:8]]
+    return out
 
 
 def benchmark(reqs, budget_bytes=200e9, max_active=16, cap_sample=32, load=0.8, seed=4, precision="mixed",
