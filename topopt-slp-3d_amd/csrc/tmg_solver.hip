@@ -1386,6 +1386,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                 const int64_t t1 = (int64_t)h->l1_ntx * h->l1_nty;
                 h->l1_zc32 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p32, 1));
                 h->l1_zc64 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p64, 1));
+                if (const char *ev = getenv("TMG_L1_ZCHUNK")) h->l1_zc32 = h->l1_zc64 = std::max(1, std::min(atoi(ev), 64));
             }
         }
         if (const char *ev = getenv("TMG_ZCHUNK")) h->zchunk = std::max(1, std::min(atoi(ev), 64));
