@@ -1354,6 +1354,79 @@ __global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const double *_
     }
 }
 
+// Coarse solve reading half of the symmetric inverse (large coarse systems):
+// 64x64 blocks (I, J), I <= J (pairs[p]); pass 1 forms u = A_IJ x_J and, off
+// the diagonal, v = A_IJ' x_I for each pair into P[p] = (u, v); pass 2 adds,
+// for block row I, the v of pairs (J, I), J < I, then the u of pairs (I, J),
+// J >= I, in that fixed order and scatters to the free DOFs. A is the full
+// (symmetric) n x n inverse; rows and columns past n read as zero.
+constexpr int CS_B = 64;
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_coarse_symv1(int64_t n, const double *__restrict__ A,
+                                                      const int2 *__restrict__ pairs,
+                                                      const int64_t *__restrict__ freeidx, const T *__restrict__ b,
+                                                      double *__restrict__ P, const int *done) {
+    if (done && *done) return;
+    const int2 pr = pairs[blockIdx.x];
+    const int64_t I0 = (int64_t)pr.x * CS_B, J0 = (int64_t)pr.y * CS_B;
+    const int c = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;   // column, group of 16 rows
+    __shared__ double xr[CS_B], xc[CS_B];
+    __shared__ double T_[CS_B][CS_B + 1];    // row products, padded
+    __shared__ double V[4][CS_B];
+    if (threadIdx.x < CS_B) {
+        const int64_t i = I0 + threadIdx.x;
+        xr[threadIdx.x] = i < n ? (double)b[freeidx[i]] : 0.0;
+    } else if (threadIdx.x < 2 * CS_B) {
+        const int64_t j = J0 + threadIdx.x - CS_B;
+        xc[threadIdx.x - CS_B] = j < n ? (double)b[freeidx[j]] : 0.0;
+    }
+    __syncthreads();
+    const int64_t col = J0 + c;
+    const bool cin = col < n;
+    const double xj = xc[c];
+    double v = 0.0;
+#pragma unroll 4
+    for (int rr = 0; rr < 16; ++rr) {
+        const int r = g * 16 + rr;
+        const int64_t row = I0 + r;
+        const double a = (cin && row < n) ? A[row * n + col] : 0.0;
+        T_[r][c] = a * xj;
+        v += a * xr[r];
+    }
+    V[g][c] = v;
+    __syncthreads();
+    double *Pp = P + (int64_t)blockIdx.x * 2 * CS_B;
+    if (threadIdx.x < CS_B) {
+        double u = 0.0;
+        for (int q = 0; q < CS_B; ++q) u += T_[threadIdx.x][q];
+        Pp[threadIdx.x] = u;
+    } else if (threadIdx.x < 2 * CS_B) {
+        const int q = (int)threadIdx.x - CS_B;
+        Pp[CS_B + q] = ((V[0][q] + V[1][q]) + V[2][q]) + V[3][q];
+    }
+}
+
+// pass 2: one wave per block row I; pair index of (I, J), I <= J, is
+// I*nb - I*(I-1)/2 + (J - I)
+template <typename T>
+__global__ __launch_bounds__(64) void k_coarse_symv2(int64_t n, int nb, const double *__restrict__ P,
+                                                     const int64_t *__restrict__ freeidx, T *__restrict__ x,
+                                                     const int *done) {
+    if (done && *done) return;
+    const int I = (int)blockIdx.x, r = (int)threadIdx.x;
+    double y = 0.0;
+    for (int J = 0; J < I; ++J) {                      // v of pair (J, I)
+        const int64_t p = (int64_t)J * nb - (int64_t)J * (J - 1) / 2 + (I - J);
+        y += P[p * 2 * CS_B + CS_B + r];
+    }
+    for (int J = I; J < nb; ++J) {                     // u of pair (I, J)
+        const int64_t p = (int64_t)I * nb - (int64_t)I * (I - 1) / 2 + (J - I);
+        y += P[p * 2 * CS_B + r];
+    }
+    const int64_t i = (int64_t)I * CS_B + r;
+    if (i < n) x[freeidx[i]] = (T)y;
+}
+
 __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,
                                               double *__restrict__ y, const int *done) {
     if (done && *done) return;

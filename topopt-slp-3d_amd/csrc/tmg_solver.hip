@@ -75,6 +75,13 @@ struct tmg_solver {
     // coarsest grid
     int64_t nc = 0;
     double *Ainv = nullptr, *cb = nullptr, *cx = nullptr;
+    // large coarse systems: solve reading the upper block triangle of Ainv
+    // (k_coarse_symv1/2); below the threshold its second pass costs more
+    // than the halved matrix read saves
+    bool coarse_symv = false;
+    int cs_nb = 0;                    // 64-row blocks
+    int2 *cs_pairs = nullptr;         // upper block pairs (I <= J)
+    double *cs_part = nullptr;        // per pair: A_IJ x_J and A_IJ' x_I
     int *d_info = nullptr;
     rocblas_handle rb = nullptr;
     // PCG
@@ -442,6 +449,14 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
+    if (h->coarse_symv) {
+        const int64_t npairs = (int64_t)h->cs_nb * (h->cs_nb + 1) / 2;
+        hipLaunchKernelGGL((k_coarse_symv1<T>), dim3((unsigned)npairs), dim3(TPB), 0, h->st, h->nc, h->Ainv,
+                           h->cs_pairs, lc.d_free, b, h->cs_part, done);
+        hipLaunchKernelGGL((k_coarse_symv2<T>), dim3((unsigned)h->cs_nb), dim3(CS_B), 0, h->st, h->nc, h->cs_nb,
+                           h->cs_part, lc.d_free, x, done);
+        return;
+    }
     hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc, h->Ainv,
                        lc.d_free, b, x, done);
 }
@@ -886,7 +901,7 @@ void destroy_all(tmg_solver *h) {
         dfree(h, lv.b); dfree(h, lv.x); dfree(h, lv.t); dfree(h, lv.d_free); dfree(h, lv.d_cidx);
     }
     dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d);
-    dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
+    dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->cs_pairs); dfree(h, h->cs_part); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
     dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
@@ -1330,6 +1345,15 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->l1_wht = wht_mr_table(K8.data(), h->kwg64, h->kwg32);
         if (h->l1_wht) wht_tiles(h->lev[1].nx, h->lev[1].ny, h->l1_tyl, h->l1_ntx, h->l1_nty);
     }
+    h->coarse_symv = h->nc >= 16384;
+    if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
+    h->cs_nb = (int)((h->nc + CS_B - 1) / CS_B);
+    std::vector<int2> cs_pairs;
+    if (h->coarse_symv) {
+        cs_pairs.reserve((size_t)h->cs_nb * (h->cs_nb + 1) / 2);
+        for (int I = 0; I < h->cs_nb; ++I)
+            for (int J = I; J < h->cs_nb; ++J) cs_pairs.push_back(make_int2(I, J));   // index I*nb - I(I-1)/2 + (J-I)
+    }
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
     if ((rc = dalloc(h, &h->K0_64, 576 * h->nsub)) || (rc = dalloc(h, &h->K0_32, 576 * h->nsub)) ||
@@ -1340,6 +1364,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
+        (rc = dalloc(h, &h->cs_pairs, h->coarse_symv ? (int64_t)h->cs_nb * (h->cs_nb + 1) / 2 : 1)) ||
+        (rc = dalloc(h, &h->cs_part, h->coarse_symv ? (int64_t)h->cs_nb * (h->cs_nb + 1) * CS_B : 1)) ||
         (rc = dalloc(h, &h->px, l0.ndof)) || (rc = dalloc(h, &h->pr, l0.ndof)) || (rc = dalloc(h, &h->pd, l0.ndof)) ||
         (rc = dalloc(h, &h->pd2, l0.ndof)) ||
         (rc = dalloc(h, &h->pq, l0.ndof)) || (rc = dalloc(h, &h->pf, l0.ndof)))
@@ -1406,6 +1432,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         hipMemcpyAsync(h->kw32_d, h->kw32.g, sizeof(h->kw32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kwg64_d, h->kwg64.g, sizeof(h->kwg64.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kwg32_d, h->kwg32.g, sizeof(h->kwg32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        (!cs_pairs.empty() && hipMemcpyAsync(h->cs_pairs, cs_pairs.data(), sizeof(int2) * cs_pairs.size(),
+                                             hipMemcpyHostToDevice, h->st) != hipSuccess) ||
         hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
