@@ -1257,28 +1257,38 @@ __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict
     else block_reduce_store(s, partial, blockIdx.x);
 }
 
-// Single-block, fixed-order final reduction and the scalar PCG recurrences.
+// Single-block, fixed-order final reduction and the scalar PCG recurrences:
+// strided per-thread sums, then wave shuffles and the 4 wave sums in order
+// (one barrier).
+constexpr int FIN_THREADS = 256;
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_finalize(int np, const double *partial, int np2, const double *partial2,
-                                                   PcgScalars *sc) {
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(int np, const double *partial, int np2, const double *partial2,
+                                                          PcgScalars *sc) {
     if (MODE != FIN_INIT && sc->done) return;
-    __shared__ double red[1024];
-    __shared__ double red2[1024];
+    __shared__ double red[FIN_THREADS / 64], red2[FIN_THREADS / 64];
     double s = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < np; i += 1024) s += partial[i];
+    for (int i = threadIdx.x; i < np; i += FIN_THREADS) s += partial[i];
     if (MODE == FIN_INIT)
-        for (int i = threadIdx.x; i < np2; i += 1024) s2 += partial2[i];
-    red[threadIdx.x] = s;
-    red2[threadIdx.x] = s2;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[threadIdx.x] += red[threadIdx.x + w];
-            red2[threadIdx.x] += red2[threadIdx.x + w];
-        }
-        __syncthreads();
+        for (int i = threadIdx.x; i < np2; i += FIN_THREADS) s2 += partial2[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        s2 += __shfl_down(s2, off, 64);
     }
-    if (threadIdx.x == 0) fin_update(MODE, red[0], red2[0], sc);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = s;
+        red2[threadIdx.x >> 6] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0, t2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < FIN_THREADS / 64; ++w) {
+            t += red[w];
+            t2 += red2[w];
+        }
+        fin_update(MODE, t, t2, sc);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -646,23 +646,23 @@ void pcg_iteration(tmg_solver *h, int parity) {
     if (fused_ok(h)) {
         // rho = r' s (partials from the post-smoothing apply, preconditioner precision)
         const int npost = (int)apply_blocks(h, 0, s32 == nullptr);
-        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, npost, h->partial, 0, nullptr, h->d_sc);
+        hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, npost, h->partial, 0, nullptr, h->d_sc);
         double *dold = parity ? h->pd2 : h->pd, *dnew = parity ? h->pd : h->pd2;
         if (s32) direction_apply<float>(h, s32, dold, dnew, done);
         else direction_apply<double>(h, s64, dold, dnew, done);
-        hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
+        hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr, h->d_sc);
         hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, dnew, h->pq, h->partial, h->d_sc,
                            finrr);
     } else {
         if (nrho > 0) {
             // rho = r' s from the post-sweep's partial sums (finished there when fin_fused)
             if (!h->fin_fused)
-                hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, nrho, h->partial, 0, nullptr,
+                hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, nrho, h->partial, 0, nullptr,
                                    h->d_sc);
         } else {
             if (s32) hipLaunchKernelGGL((k_dot<double, float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s32, h->partial, done);
             else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
-            hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+            hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
         }
         if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
         else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
@@ -673,14 +673,14 @@ void pcg_iteration(tmg_solver *h, int parity) {
             launch_ebe<double, double, SRC_X, EPI_DOT>(h, a);
         } else {
             apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);      // q = A d, d'q
-            hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(1024), 0, h->st, napply, h->partial, 0, nullptr,
+            hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr,
                                h->d_sc);
         }
         hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial,
                            h->d_sc, finrr);
     }
     if (!h->fin_fused)
-        hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
+        hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
 }
 
 int check_launch(tmg_solver *h) {
@@ -734,7 +734,7 @@ int solve_begin(tmg_solver *h) {
     const unsigned gdot = (unsigned)std::min<int64_t>(nblk(n), h->npartial);
     hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pr, h->partial, nullptr);
     hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pf, h->pf, h->partial2, nullptr);
-    hipLaunchKernelGGL(k_finalize<FIN_INIT>, dim3(1), dim3(1024), 0, h->st, (int)gdot, h->partial, (int)gdot,
+    hipLaunchKernelGGL(k_finalize<FIN_INIT>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, (int)gdot,
                        h->partial2, h->d_sc);
     TRY(check_launch(h));
     if (h->use_graph && !h->graph_exec) {
