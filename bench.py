@@ -45,6 +45,13 @@ def parse():
                     help="solve: MG-PCG solves + serving stream (default); slp: the multiresolution SLP loop")
     ap.add_argument("--slp-iters", type=int, default=30)
     ap.add_argument("--slp-grid", default="240x120x120", help="displacement mesh of the SLP workload")
+    ap.add_argument("--probe-kernel", type=int, default=None,
+                    help="profiling aid: only launch this solver kernel (tmg_time_kernel id) --kernel-reps times "
+                         "on the workload and print its average; no JSON bench line")
+    ap.add_argument("--probe-precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="profiling aid: only run an elementwise kernel that reads and writes this many bytes "
+                         "(calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -60,41 +67,115 @@ def profile_traffic(name):
     return None
 
 
-def cpu_baseline():
-    """FP64 CPU restatement of the reference's assembled GMG-PCG path (the
-    oracle) timed on this host: solve phase of the 96x48x48 cantilever."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(prob, x, L, gpu_iters):
+    """FP64 CPU restatement of the reference's assembled GMG-PCG path (the
+    oracle) timed on this host, on the benchmark's own workload: setup
+    (assembly + Galerkin + coarse Cholesky) and one full solve on every
+    allotted core, then a bounded one-core sample (2 PCG iterations from
+    the same start)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    from tmg import problem as P
-    cores = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    prob = P.cantilever(96, 48, 48)
-    x = P.random_density(96, 48, 48, 2)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    prev = oracle.set_threads(cores)
     o = oracle.Oracle(prob)
     t0 = time.perf_counter()
     o.assemble(x)
-    o.setup_gmg(4, 0.6)
+    o.setup_gmg(L, 0.6)
     t_setup = time.perf_counter() - t0
-    times, its = [], []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        u, it, rr = o.solve(prob.f)
-        times.append(time.perf_counter() - t0)
-        its.append(it)
-    t = statistics.median(times)
-    it = its[0]
+    t0 = time.perf_counter()
+    u, it, rr = o.solve(prob.f)
+    t_all = time.perf_counter() - t0
+    oracle.set_threads(1)
+    k1 = 2
+    t0 = time.perf_counter()
+    o.solve(prob.f, maxiter=k1)
+    t_one = time.perf_counter() - t0
+    oracle.set_threads(prev)
+    v_all = prob.nfree * (it - 1) / t_all
+    v_one = prob.nfree * k1 / t_one
     return {
-        "value": prob.nfree * (it - 1) / t,
+        "value": v_all,
         "unit": "DOF-iters/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "cores": cores,
         "kind": "port",
-        "sample": (f"FP64 C/OpenMP restatement of the reference's assembled SolveMG path on the 96x48x48 "
-                   f"cantilever ({prob.nfree} free DOFs, filtered random densities seed 2, 4-level V(1,1) "
-                   f"Jacobi omega 0.6, tol 1e-8); solve phase only, median of 3: {t:.3f} s, {it - 1} PCG "
-                   f"iterations; setup (assembly + Galerkin + Cholesky) {t_setup:.2f} s"),
+        "one_core_value": v_one,
+        "sample": (f"FP64 C/OpenMP restatement of the reference's assembled SolveMG path (the oracle; baseline only, "
+                   f"not MATLAB) on the benchmark's workload ({prob.nelx}x{prob.nely}x{prob.nelz}, {prob.nfree} "
+                   f"free DOFs, {L}-level V(1,1) Jacobi, tol 1e-8) on {cores} threads of a {cpu_model()}: one full "
+                   f"solve, {it - 1} PCG iterations (GPU FP64: {gpu_iters - 1}), {t_all:.2f} s; setup (assembly + "
+                   f"Galerkin + coarse Cholesky) {t_setup:.2f} s; one-core sample of {k1} PCG iterations "
+                   f"{t_one:.2f} s = {v_one:.3e} DOF-iters/s"),
     }
+
+
+def iteration_bytes(prob, L, prec):
+    """Algorithmic HBM bytes of one PCG iteration (each array the kernels read
+    or write counted once per pass; no halo or cache effects), for the
+    solver's schedule: FP64 outer recurrences and q = K d; the fine level and
+    level 1 matrix-free (moduli only); assembled symmetric stencils (126
+    coefficients per node) below; the dense FP64 coarse inverse."""
+    s = 4 if prec == "mixed" else 8
+    n0, ne0 = prob.ndof, prob.nelem
+    b = 8 * (2 * n0 + ne0)            # q = K d (read d, E; write q)
+    b += 8 * 6 * n0                   # x, r update with ||r||^2 (read x, r, d, q; write x, r)
+    b += (s + 16) * n0                # d = s + beta d
+    b += (8 + 2 * s) * n0             # first fine sweep from zero (read r, invD; write x)
+    b += s * (3 * n0 + ne0) + 8 * n0  # fine residual (read x, E, r; write)
+    b += (s * 4 + 8) * n0 + s * ne0   # fine post-sweep (read x, invD, E, r; write)
+    ex, ey, ez = prob.nelx, prob.nely, prob.nelz
+    nprev = n0
+    for l in range(1, L):
+        ex, ey, ez = ex // 2, ey // 2, ez // 2
+        n = 3 * (ex + 1) * (ey + 1) * (ez + 1)
+        b += s * (nprev + n)          # restriction into level l
+        b += s * (2 * nprev + n)      # prolongation back onto level l - 1
+        if l == L - 1:
+            b += 8 * n * n + 2 * s * n     # dense coarse inverse times the coarse residual
+        elif l == 1:
+            b += s * (3 * n + 8 * ex * ey * ez) * 2 + s * 3 * n   # two matrix-free sweeps + sweep from zero
+        else:
+            b += 2 * s * (42 * n + 3 * n)  # two stencil sweeps (126 coefficients per node = 42 per DOF)
+        nprev = n
+    return b
+
+
+def probe(args):
+    """Profiling aid: launch one solver kernel (or a copy of known size)
+    --kernel-reps times; prints its HIP-event average. Kernel ids are those of
+    tmg_time_kernel (0 FP64 fine apply q = K d, 1 fine Jacobi sweep, 2 level-2
+    sweep, 3 V-cycle, 4/5 fused fine residual / post-sweep, 6 fused direction
+    update + apply)."""
+    import torch
+    torch.cuda.init()
+    if args.copy_bytes:
+        n = args.copy_bytes // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.kernel_reps):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        log(json.dumps({"copy_bytes": args.copy_bytes, "reps": args.kernel_reps}))
+        return
+    import tmg
+    from tmg import problem as P
+    nelx, nely, nelz = (int(v) for v in args.grid.split("x"))
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, args.seed)
+    s = tmg.Solver(prob, args.levels, precision=args.probe_precision)
+    s.update_density(x)
+    ms = s.time_kernel(args.probe_kernel, args.kernel_reps)
+    log(json.dumps({"kernel": args.probe_kernel, "precision": args.probe_precision, "avg_ms": ms,
+                    "reps": args.kernel_reps, "n_dof": prob.ndof, "n_el": prob.nelem}))
 
 
 def slp_workload(args):
@@ -144,6 +225,9 @@ def main():
     if args.workload == "slp":
         slp_workload(args)
         return
+    if args.probe_kernel is not None or args.copy_bytes:
+        probe(args)
+        return
     import numpy as np
     import torch
 
@@ -158,10 +242,16 @@ def main():
     prob = P.cantilever(nelx, nely, nelz)
     x = P.random_density(nelx, nely, nelz, args.seed)
     f_full = torch.from_numpy(prob.full(prob.f)).to(dev)
-    golden = {}
+    # the FP64 oracle's solve of this workload (minted in the build container
+    # by tests/golden/make_golden.py); every bench line states each variant's
+    # error against it
     gpath = os.path.join(ROOT, "tests", "golden", f"oracle_{nelx}x{nely}x{nelz}_seed{args.seed}.json")
-    if os.path.exists(gpath):
-        golden = json.load(open(gpath))
+    if not os.path.exists(gpath):
+        raise SystemExit(f"bench.py: no oracle fixture {gpath}; mint it with tests/golden/make_golden.py --only "
+                         f"{nelx}x{nely}x{nelz}")
+    golden = json.load(open(gpath))
+    gidx = np.asarray(golden["u_sample_index"])
+    gu = np.asarray(golden["u_sample"])
 
     results = {}
     solvers = {}
@@ -196,15 +286,18 @@ def main():
         relres_true = float(np.linalg.norm(prob.f - Ku) / np.linalg.norm(prob.f))
         C = float(prob.f @ u)
         t_solve = sum(times) / len(times)
+        bytes_it = iteration_bytes(prob, L, prec)
         results[prec] = dict(it=its[-1], relres=rr, relres_true=relres_true, compliance=C, t_solve=t_solve,
                              t_setup_structure=t1 - t0, t_setup_operator=t2 - t1,
-                             dof_iters=prob.nfree * (its[-1] - 1) / t_solve)
+                             dof_iters=prob.nfree * (its[-1] - 1) / t_solve,
+                             it_oracle=golden["it"],
+                             compliance_rel_err_vs_oracle=abs(C - golden["compliance"]) / abs(golden["compliance"]),
+                             u_sample_rel_err_vs_oracle=float(np.linalg.norm(u[gidx] - gu) / np.linalg.norm(gu)),
+                             algorithmic_bytes_per_iteration=bytes_it,
+                             solve_algorithmic_gbs=bytes_it * (its[-1] - 1) / t_solve / 1e9)
         log(f"{prec}: it={its[-1]} t_solve={t_solve * 1e3:.2f} ms relres={rr:.2e} true={relres_true:.2e} C={C:.10e} "
             f"setup={t1 - t0:.3f}+{t2 - t1:.3f} s DOF-iters/s={results[prec]['dof_iters']:.3e}")
 
-    for prec in results:
-        if "compliance" in golden:
-            results[prec]["compliance_rel_err_vs_oracle"] = abs(results[prec]["compliance"] - golden["compliance"]) / abs(golden["compliance"])
     results["mixed"]["compliance_rel_err_vs_fp64"] = abs(results["mixed"]["compliance"] - results["fp64"]["compliance"]) / abs(results["fp64"]["compliance"])
 
     # dominant kernels, HIP events on the solver's stream
@@ -264,7 +357,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(prob, x, L, results["fp64"]["it"])
         log("cpu:", json.dumps(cpu))
     mixed, fp64 = results["mixed"], results["fp64"]
     line = {

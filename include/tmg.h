@@ -191,6 +191,10 @@ int tmg_lp_volume(int64_t n, const double *c, const double *a, const double *sL,
  * tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
+/* Instrumentation (no counterpart in the reference): device bytes the handle
+ * holds, each allocation rounded up to 256 bytes. */
+int64_t tmg_device_bytes(const tmg_solver *h);
+
 /* ---------------------------------------------------------------------------
  * Serving (no counterpart in the reference, which solves one system at a time
  * inside its SLP loop): a continuous-batching server for streams of
@@ -216,7 +220,8 @@ int tmg_server_wait(tmg_server *s, int64_t id, double timeout_s, int *state);
  * times[4] = {arrival, admission, start of PCG, completion} on the server clock. */
 int tmg_server_result(tmg_server *s, int64_t id, double *u, int *it, double *relres, double *times);
 int tmg_server_release(tmg_server *s, int64_t id);
-/* stats[6] = {bytes reserved, peak bytes reserved, queued, running, done, clock}. */
+/* stats[7] = {bytes reserved, peak bytes reserved, queued, running, done, clock,
+ * peak number of requests solving at the same time}. */
 int tmg_server_stats(tmg_server *s, double *stats);
 double tmg_server_clock(const tmg_server *s);
 /* Device bytes the server reserves for a request of this shape. */

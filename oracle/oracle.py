@@ -63,6 +63,8 @@ def lib():
         L.or_level_dims.argtypes = [_p, _i, _p]
         L.or_ngrids.restype = _i
         L.or_ngrids.argtypes = [_p]
+        L.or_set_threads.restype = _i
+        L.or_set_threads.argtypes = [_i]
         L.or_jacobi_diag.argtypes = [_p, _i, _p]
         L.or_filter_build.restype = _i64
         L.or_filter_build.argtypes = [_i, _i, _i, _d, _d, _d, _d, _i, _p, _p, _p, _p, _p]
@@ -71,6 +73,11 @@ def lib():
         L.or_grad0_mr.argtypes = [_p, _i, _p, _p, _p]
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's following calls; returns the previous count."""
+    return int(lib().or_set_threads(int(n)))
 
 
 def _ptr(a):

@@ -37,6 +37,21 @@ def lp_highs(c, A, rhs, sL, sU, equality=False):
     return (r.x if r.status == 0 else None), float(lam), r.status == 0
 
 
+def fallback_lp_highs(A, b, sL2, sU2, equality=False):
+    """The reference's LP for an infeasible trust-region step
+    (StrSLP.m:271-285): variables [s; t1; t2], cost [0; 1; 1], row
+    [A -1 1] (<= or = -b), bounds [sL2; 0; 0] .. [sU2; inf; inf], solved by
+    HiGHS' dual simplex as stated. Returns (s, lambda of the row)."""
+    n = A.size
+    c2 = np.concatenate([np.zeros(n), [1.0, 1.0]])
+    A2 = np.concatenate([A, [-1.0, 1.0]])
+    lo = np.concatenate([sL2, [0.0, 0.0]])
+    hi = np.concatenate([sU2, [np.inf, np.inf]])
+    s2, lam, ok = lp_highs(c2, A2, -b, lo, hi, equality=equality)
+    assert ok, "the fallback LP is always feasible"
+    return s2[:n], lam
+
+
 def str_slp(prob, volfrac, p=3.0, emin=1e-9, rmin=1.5, opfilter=2, ngrids=3, mr=1, x0=None, maxiter=10,
             delta0=0.1, eta=0.1, rho=0.5, alphaR=0.25, alphaA=2.0, tolS=1e-4, tolG=1e-3, tolF=5e-2,
             maxcount=3):
@@ -90,8 +105,7 @@ def str_slp(prob, volfrac, p=3.0, emin=1e-9, rmin=1.5, opfilter=2, ngrids=3, mr=
         s, lam, ok = lp_highs(Fgrad, A, -b, sL, sU)
         if not ok:                                          # StrSLP.m:271-285
             sL2, sU2 = np.maximum(-0.8 * delta, -x), np.minimum(0.8 * delta, 1 - x)
-            s = np.where(A > 0, sL2, sU2)
-            lam = 1.0
+            s, lam = fallback_lp_highs(A, b, sL2, sU2)
         fval = float(Fgrad @ s)
         snorm = float(np.abs(s).max())
         xold, bold, deltaold = x, b, delta

@@ -344,6 +344,14 @@ void or_destroy(or_ctx *c) {
 int64_t or_nfree(const or_ctx *c, int level) { return c->lev[level].nfree; }
 int64_t or_ndof(const or_ctx *c, int level) { return c->lev[level].ndof; }
 int or_ngrids(const or_ctx *c) { return c->L; }
+
+/* OpenMP threads of the calls that follow (the CPU baseline's all-core and
+ * one-core legs); returns the previous count. */
+int or_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
 void or_level_dims(const or_ctx *c, int level, int *nel3) {
     nel3[0] = c->lev[level].nelx; nel3[1] = c->lev[level].nely; nel3[2] = c->lev[level].nelz;
 }

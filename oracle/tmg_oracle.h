@@ -77,6 +77,8 @@ void or_csr_export(const or_ctx *c, int which, int level, int64_t *rp, int32_t *
 void or_level_free(const or_ctx *c, int level, int64_t *full_of_free);
 void or_level_dims(const or_ctx *c, int level, int *nel3);
 int or_ngrids(const or_ctx *c);
+/* OpenMP threads of the following calls; returns the previous count. */
+int or_set_threads(int n);
 /* Jacobi diagonal M{l} = (1/omega) diag(A{l}). */
 void or_jacobi_diag(const or_ctx *c, int level, double *m);
 

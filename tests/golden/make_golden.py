@@ -102,10 +102,14 @@ def large(grid, seed, levels):
     u, it, rr = o.solve(prob.f)
     t2 = time.time()
     r = prob.f - o.spmv(0, u)
+    # a fixed sample of u: 256 evenly spaced free DOFs and every loaded DOF
+    idx = np.unique(np.concatenate([np.linspace(0, prob.nfree - 1, 256).astype(np.int64),
+                                    np.nonzero(prob.f)[0]]))
     rec = {"grid": list(grid), "seed": seed, "levels": levels, "nfree": int(prob.nfree), "it": int(it),
            "relres": float(rr), "true_relres": float(np.linalg.norm(r) / np.linalg.norm(prob.f)),
            "compliance": float(prob.f @ u), "u_norm": float(np.linalg.norm(u)),
            "u_tip_mean_y": float(np.mean(u[np.nonzero(prob.f)[0]])),
+           "u_sample_index": idx.tolist(), "u_sample": u[idx].tolist(),
            "setup_s": t1 - t0, "solve_s": t2 - t1, "omp_threads": os.environ.get("OMP_NUM_THREADS")}
     name = f"oracle_{nelx}x{nely}x{nelz}_seed{seed}.json"
     with open(os.path.join(HERE, name), "w") as fh:
@@ -115,8 +119,13 @@ def large(grid, seed, levels):
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--large", action="store_true")
+    ap.add_argument("--large", action="store_true", help="also the 96x48x48 and 192x96x96 oracle solves")
+    ap.add_argument("--only", default=None, help="only this large grid (e.g. 192x96x96)")
     a = ap.parse_args()
+    if a.only:
+        g = tuple(int(v) for v in a.only.split("x"))
+        large(g, {96: 2, 192: 3}[g[0]], {96: 4, 192: 5}[g[0]])
+        sys.exit(0)
     setup_fixtures()
     small_cantilever()
     if a.large:

@@ -432,3 +432,35 @@ def test_multiresolution_spectral_matches_stencil(monkeypatch, grid):
         assert _rel(out["wht", "fp64"][i], out["stencil", "fp64"][i]) < tol, i
     assert _rel(out["wht", "mixed"][2], out["stencil", "mixed"][2]) < 1e-5
     assert _rel(out["wht", "mixed"][2], out["wht", "fp64"][2]) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", [(16, 8, 8, 3, 21), (24, 12, 12, 3, 22)])
+def test_symmetric_coarse_solve_matches_full_and_oracle(oracle_mod, monkeypatch, precision, nelx, nely, nelz, L,
+                                                        seed):
+    """The coarse solve that reads only the upper block triangle of the
+    symmetric inverse (k_coarse_symv1/2, the default from 16,384 coarse DOFs)
+    forced on small coarse systems whose size is not a multiple of the 64-row
+    block (108 and 288 DOFs): the V-cycle equals the full-matrix solve's to
+    rounding, and the PCG solve meets the oracle."""
+    _say(f"symmetric coarse solve {precision} {nelx}x{nely}x{nelz} L={L}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    monkeypatch.setenv("TMG_COARSE_SYMV", "0")
+    s0 = _setup(prob, x, L, precision)
+    monkeypatch.setenv("TMG_COARSE_SYMV", "1")
+    s1 = _setup(prob, x, L, precision)
+    nc = s1.level_info(L)[1].size
+    assert nc % 64 != 0
+    b = np.random.default_rng(seed).standard_normal(prob.nfree)
+    v0, v1 = s0.vcycle(b), s1.vcycle(b)
+    assert _rel(v1, v0) < (1e-13 if precision == "fp64" else 1e-6)
+    if precision == "fp64":
+        assert _rel(v1, o.vcycle(b)) < 1e-8
+    u_or, it_or, _ = o.solve(prob.f)
+    u, it, rr, rc = s1.solve(prob.f)
+    assert rc == 0 and rr <= 1e-8
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)

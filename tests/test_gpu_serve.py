@@ -46,9 +46,8 @@ def test_same_alone_and_in_the_mix():
         assert a["it"] == m["it"]
         assert np.array_equal(a["u"], m["u"]), r["i"]
     assert st["peak_bytes_reserved"] <= 8e9
-    # overlap: in the mix, two requests were running at the same time
-    iv = [(m["t_start"], m["t_done"]) for m in mix]
-    assert any(a0 < b1 and b0 < a1 for i, (a0, a1) in enumerate(iv) for (b0, b1) in iv[i + 1:])
+    # overlap: the server saw at least two requests solving at the same time
+    assert st["peak_running"] >= 2
 
 
 def test_server_matches_direct_solver():

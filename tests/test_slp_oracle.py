@@ -72,3 +72,49 @@ def test_product_lp_equals_highs_on_slp_instances(oracle_mod):
         if ok:
             assert abs(g @ s - g @ s_h) <= 1e-10 * abs(g @ s_h)
             assert abs(lam - lam_h) <= 1e-8 * max(abs(lam_h), 1e-30) + 1e-300
+
+
+@pytest.mark.parametrize("case", ["ineq_above", "eq_above", "eq_below", "eq_mixed_signs_below"])
+def test_feasibility_step_matches_fallback_lp(oracle_mod, case):
+    """The infeasible-LP fallback of StrSLP.m:271-285 (min t1 + t2 with
+    [A -1 1] s = -b or <= -b) solved by HiGHS in the oracle, against the
+    product's closed form: the least-violation box point, with the direction
+    of the violation (an equality row whose target lies above every reachable
+    volume must increase the volume, not decrease it)."""
+    import slp_oracle
+    from tmg.slp import feasibility_step
+    rng = np.random.default_rng(["ineq_above", "eq_above", "eq_below", "eq_mixed_signs_below"].index(case) + 40)
+    n = 400
+    a = 0.5 + rng.random(n)
+    if case == "eq_mixed_signs_below":
+        a *= np.where(rng.random(n) < 0.3, -1.0, 1.0)
+    x = rng.random(n)
+    delta = 0.05
+    sL2, sU2 = np.maximum(-0.8 * delta, -x), np.minimum(0.8 * delta, 1 - x)
+    lo = float(np.where(a > 0, a * sL2, a * sU2).sum())
+    hi = float(np.where(a > 0, a * sU2, a * sL2).sum())
+    equality = case.startswith("eq")
+    # b places the row target -b outside [lo, hi]
+    b = -(lo - 5.0) if case.endswith("above") else -(hi + 5.0)
+    s_ref, lam_ref = slp_oracle.fallback_lp_highs(a, b, sL2, sU2, equality=equality)
+    s, lam = feasibility_step(a, b, sL2, sU2, equality=equality)
+    np.testing.assert_allclose(s, s_ref, rtol=0, atol=1e-12)
+    assert lam == pytest.approx(lam_ref, abs=1e-9)
+    if case.endswith("below"):
+        assert a @ s > 0 and lam == -1.0      # the volume grows toward the target
+    else:
+        assert a @ s < 0 and lam == 1.0
+
+
+def test_feasibility_step_when_the_box_meets_the_row():
+    from tmg.slp import feasibility_step
+    rng = np.random.default_rng(4)
+    a = 0.5 + rng.random(50)
+    sL2, sU2 = -0.04 * np.ones(50), 0.04 * np.ones(50)
+    for b, eq in ((0.2 * 0.04 * a.sum(), True), (-0.2 * 0.04 * a.sum(), True), (0.3, False), (-0.3, False)):
+        s, lam = feasibility_step(a, b, sL2, sU2, equality=eq)
+        assert lam == 0.0 and np.all(s >= sL2 - 1e-15) and np.all(s <= sU2 + 1e-15)
+        if eq:
+            assert abs(a @ s + b) <= 1e-12
+        else:
+            assert a @ s <= -b + 1e-12

@@ -1451,6 +1451,36 @@ __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restric
     if (lane == 0) y[row] = s;
 }
 
+// Checksum of n 64-bit words (their sum modulo 2^64, so the order of the
+// additions does not matter): per-block sums of a block-strided range into
+// part[block], then one block adds the nb partials into part[nb].
+__device__ inline uint64_t block_sum_u64(uint64_t v) {
+    __shared__ uint64_t red[TPB / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < TPB / 64; ++w) s += red[w];
+    return s;
+}
+
+__global__ __launch_bounds__(TPB) void k_checksum_partial(int64_t n, const uint64_t *__restrict__ v,
+                                                          uint64_t *__restrict__ part) {
+    uint64_t s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) s += v[i];
+    s = block_sum_u64(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(TPB) void k_checksum_final(int nb, uint64_t *__restrict__ part) {
+    uint64_t s = 0;
+    for (int i = threadIdx.x; i < nb; i += TPB) s += part[i];
+    s = block_sum_u64(s);
+    if (threadIdx.x == 0) part[nb] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Setup: moduli, diagonals, Galerkin element matrices, stencils, dense coarse.
 // ---------------------------------------------------------------------------

@@ -3,14 +3,17 @@
 // solver through its internal begin / launch / poll steps).
 //
 // Requests are admitted FIFO while a slot is free and the estimated device
-// memory of the request fits the budget. Each slot owns a HIP stream and a
-// pinned scalar block; device memory comes from one stream-ordered pool, so
-// admitting and retiring requests never synchronises the device. A setup
+// memory of the request fits the budget. Each slot owns a HIP stream, a
+// pinned scalar block, a pinned staging buffer for the request's host vectors
+// and a stream-ordered memory pool that never hands memory to another stream,
+// so admitting and retiring requests never synchronises the device and no
+// request can be given memory another request's kernels still use. A setup
 // thread builds each admitted hierarchy (Galerkin refresh, coarse factor,
-// graph capture) while a driver thread keeps every running solve supplied
-// with chunks of PCG iterations; kernels of different requests overlap on the
-// device. Results are bitwise identical to a solve run alone: every kernel is
-// deterministic and a request's launches depend only on its own problem.
+// graph capture) and checks that the device holds the request's densities;
+// a driver thread keeps every running solve supplied with chunks of PCG
+// iterations; kernels of different requests overlap on the device. Results
+// are bitwise identical to a solve run alone: every kernel is deterministic
+// and a request's launches depend only on its own problem.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -75,9 +78,21 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
     add(8 * ne0); add(4 * ne0);                                // level-1 children's moduli (upper bound)
     add(8 * ne1 * 576); add(8 * (L > 2 ? ne2 * 576 : 1));
     add(8 * ncoarse * ncoarse); add(8 * ncoarse); add(8 * ncoarse); add(4);
+    // symmetric coarse solve (coarse systems of 16,384 DOFs and more): block
+    // pairs and per-pair partial products (upper bound: every coarse DOF free)
+    if (ncoarse >= 16384) {
+        const int64_t nb = (ncoarse + CS_B - 1) / CS_B;
+        add(sizeof(int2) * (size_t)(nb * (nb + 1) / 2));
+        add(8 * (size_t)(nb * (nb + 1) * CS_B));
+    }
     for (int k = 0; k < 6; ++k) add(8 * ndof0);
-    const int64_t pb = (nn0x * nn0y + TPB - 1) / TPB;
-    const int64_t nblocks = std::max<int64_t>(pb * nn0z, 4096);  // zchunk >= 1 bound
+    // per-block partial sums: z chunks of at least one plane, of either the
+    // 256-node blocks of the dense apply or the overlapping tiles of the
+    // spectral apply (the larger count)
+    int tyl = 4, ntx = 1, nty = 1;
+    wht_tiles((int)nn0x, (int)nn0y, tyl, ntx, nty);
+    const int64_t pb = std::max<int64_t>((nn0x * nn0y + TPB - 1) / TPB, (int64_t)ntx * nty);
+    const int64_t nblocks = std::max<int64_t>(pb * nn0z, 4096);
     add(8 * nblocks); add(8 * nblocks); add(sizeof(PcgScalars));
     add(8 * nfree);                                            // upload staging
     return b + (32u << 20);                                    // rocSOLVER workspace headroom
@@ -88,11 +103,18 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
 struct tmg_server {
     size_t budget = 0, used = 0, peak = 0;
     int nslots = 0;
+    int peak_running = 0;          // most requests solving at the same time
+    // per slot: stream, pinned scalar block, pinned staging for the
+    // request's host vectors, and its own stream-ordered memory pool (device
+    // memory is reused only within a slot, i.e. only on the slot's stream)
     std::vector<hipStream_t> streams;
     std::vector<PcgScalars *> hsc;
+    std::vector<double *> stage;
+    std::vector<size_t> stage_cap;
+    std::vector<hipMemPool_t> pools;
     std::vector<char> slot_busy;
-    rocblas_handle rb = nullptr;   // unused (kept zero)
-    hipMemPool_t pool = nullptr;
+    // the coarse factorisations of concurrent set-ups run one at a time
+    std::mutex factor_mu;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<SrvReq *> queue;
@@ -130,6 +152,39 @@ void srv_release(tmg_server *S, SrvReq *r) {
     S->used -= r->est;
 }
 
+// The densities the handle holds in device memory are the ones the request
+// brought (compared through an order-independent checksum of their bits).
+int srv_verify_upload(tmg_solver *h, const double *d, const double *hv, int64_t n, const char *what) {
+    uint64_t dev = 0;
+    TRY(device_checksum(h, d, n, &dev));
+    const uint64_t host = host_checksum(hv, n);
+    if (dev != host) {
+        h->err = std::string("the ") + what + " in device memory differ from the request's (checksum " +
+                 std::to_string(dev) + " vs " + std::to_string(host) + ")";
+        return TMG_ERR_HIP;
+    }
+    return 0;
+}
+
+// A failed coarse factorisation is reported with what tells a corrupted
+// set-up from a singular operator: whether the device still holds the
+// request's densities, and whether building the same hierarchy again on the
+// slot's stream, with nothing else in flight on it, factorises.
+void srv_diagnose_not_spd(tmg_server *S, tmg_solver *h, SrvReq *r) {
+    std::string first = h->err;
+    uint64_t dev = 0;
+    std::string dens = device_checksum(h, h->xdens, (int64_t)r->x.size(), &dev) == 0
+                           ? (dev == host_checksum(r->x.data(), (int64_t)r->x.size()) ? "intact" : "corrupted")
+                           : "unreadable";
+    int rc2;
+    {
+        std::lock_guard<std::mutex> fk(S->factor_mu);
+        rc2 = refresh(h);
+    }
+    h->err = first + "; densities on the device " + dens + "; rebuilding the same hierarchy " +
+             (rc2 == 0 ? "factorised (the first build was corrupted)" : "failed again: " + h->err);
+}
+
 void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
     (void)hipSetDevice(0);
     for (;;) {
@@ -154,17 +209,44 @@ void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
             r->state = 1;
             r->t_admit = S->now();
         }
+        const int slot = r->slot;
+        int rc = 0;
+        std::string msg;
+        // pinned staging large enough for the densities and the free vectors
+        const size_t need = std::max(r->x.size(), r->f.size());
+        if (S->stage_cap[slot] < need) {
+            if (S->stage[slot]) (void)hipHostFree(S->stage[slot]);
+            S->stage[slot] = nullptr;
+            S->stage_cap[slot] = 0;
+            const size_t cap = need + need / 4;
+            if (hipHostMalloc((void **)&S->stage[slot], cap * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+                S->stage[slot] = nullptr;
+                rc = TMG_ERR_OOM;
+                msg = "pinned staging allocation failed";
+            } else {
+                S->stage_cap[slot] = cap;
+            }
+        }
         ExtRes ext;
-        ext.st = S->streams[r->slot];
+        ext.st = S->streams[slot];
         ext.rb = rb;
-        ext.h_sc = S->hsc[r->slot];
-        ext.pool = S->pool;
+        ext.h_sc = S->hsc[slot];
+        ext.pool = S->pools[slot];
+        ext.stage = S->stage[slot];
+        ext.stage_cap = S->stage_cap[slot];
         r->prob.freedofs = r->freedofs.data();
         tmg_solver *h = nullptr;
-        int rc = setup_impl(&r->prob, &r->mg, r->prec, &ext, &h);
-        std::string msg = rc ? g_setup_err : "";
         if (!rc) {
-            rc = tmg_update_density(h, r->x.data());
+            rc = setup_impl(&r->prob, &r->mg, r->prec, &ext, &h);
+            if (rc) msg = g_setup_err;
+        }
+        if (!rc) {
+            {
+                std::lock_guard<std::mutex> fk(S->factor_mu);
+                rc = tmg_update_density(h, r->x.data());
+            }
+            if (!rc) rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
+            if (rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
             if (!rc && (rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) rc = upload_free(h, 0, r->f.data(), h->pf);
             if (!rc) rc = solve_begin(h);
             if (rc) msg = h->err;
@@ -179,6 +261,7 @@ void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
             r->state = 2;
             r->t_start = S->now();
             S->running.push_back(r);
+            S->peak_running = std::max(S->peak_running, (int)S->running.size());
         }
         S->cv.notify_all();
     }
@@ -260,18 +343,10 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
         for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
         for (PcgScalars *p : S->hsc) (void)hipHostFree(p);
         for (rocblas_handle h : S->rbs) rocblas_destroy_handle(h);
-        if (S->pool) (void)hipMemPoolDestroy(S->pool);
+        for (hipMemPool_t pl : S->pools) (void)hipMemPoolDestroy(pl);
         delete S;
         return TMG_ERR_HIP;
     };
-    hipMemPoolProps props;
-    memset(&props, 0, sizeof(props));
-    props.allocType = hipMemAllocationTypePinned;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = 0;
-    if (hipMemPoolCreate(&S->pool, &props) != hipSuccess) return bad("hipMemPoolCreate failed");
-    uint64_t keep = UINT64_MAX;  // retain freed memory in the pool
-    (void)hipMemPoolSetAttribute(S->pool, hipMemPoolAttrReleaseThreshold, &keep);
     for (int i = 0; i < max_active; ++i) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return bad("hipStreamCreate failed");
@@ -279,7 +354,22 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
         PcgScalars *p = nullptr;
         if (hipHostMalloc((void **)&p, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess) return bad("hipHostMalloc failed");
         S->hsc.push_back(p);
+        hipMemPoolProps props;
+        memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = 0;
+        hipMemPool_t pl = nullptr;
+        if (hipMemPoolCreate(&pl, &props) != hipSuccess) return bad("hipMemPoolCreate failed");
+        S->pools.push_back(pl);
+        uint64_t keep = UINT64_MAX;  // retain freed memory in the slot's pool
+        (void)hipMemPoolSetAttribute(pl, hipMemPoolAttrReleaseThreshold, &keep);
+        int off = 0;                 // never hand memory to another stream
+        (void)hipMemPoolSetAttribute(pl, hipMemPoolReuseAllowOpportunistic, &off);
+        (void)hipMemPoolSetAttribute(pl, hipMemPoolReuseAllowInternalDependencies, &off);
     }
+    S->stage.assign(max_active, nullptr);
+    S->stage_cap.assign(max_active, 0);
     S->slot_busy.assign(max_active, 0);
     const int nsetup = std::min(max_active, SRV_SETUP_THREADS);
     for (int i = 0; i < nsetup; ++i) {
@@ -377,6 +467,7 @@ int tmg_server_stats(tmg_server *S, double *stats) {
     stats[3] = (double)S->running.size();
     stats[4] = (double)done;
     stats[5] = S->now();
+    stats[6] = (double)S->peak_running;
     return TMG_OK;
 }
 
@@ -405,8 +496,10 @@ void tmg_server_destroy(tmg_server *S) {
     for (auto &kv : S->all) delete kv.second;
     for (hipStream_t st : S->streams) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     for (PcgScalars *p : S->hsc) (void)hipHostFree(p);
+    for (double *p : S->stage)
+        if (p) (void)hipHostFree(p);
     for (rocblas_handle h : S->rbs) rocblas_destroy_handle(h);
-    if (S->pool) (void)hipMemPoolDestroy(S->pool);
+    for (hipMemPool_t pl : S->pools) (void)hipMemPoolDestroy(pl);
     delete S;
 }
 

@@ -146,6 +146,8 @@ struct tmg_solver {
     // stream-ordered pool, and the stream, rocBLAS handle and pinned scalar
     // block are lent, so nothing here synchronises the whole device
     hipMemPool_t pool = nullptr;
+    double *stage = nullptr;          // pinned host staging lent by a server slot: host vectors pass through it
+    size_t stage_cap = 0;             // its capacity in doubles
     bool own_stream = true, own_rb = true, own_hsc = true;
     size_t bytes = 0;                 // device bytes held by this handle
     bool have_op = false;
@@ -173,6 +175,14 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
 inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// TMG_DEBUG_POISON=1: every new device buffer starts as 0xFF bytes (NaN in
+// either float view, -1 as an index), so a read before the first write shows
+// up in the results instead of reading whatever the memory held before
+bool poison_buffers() {
+    static const bool on = getenv_flag("TMG_DEBUG_POISON");
+    return on;
+}
+
 template <typename T>
 int dalloc(tmg_solver *h, T **p, int64_t n) {
     const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
@@ -182,6 +192,7 @@ int dalloc(tmg_solver *h, T **p, int64_t n) {
         h->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
         return TMG_ERR_OOM;
     }
+    if (poison_buffers() && h->st) (void)hipMemsetAsync((void *)*p, 0xFF, bytes, h->st);
     h->bytes += round256(bytes);
     return 0;
 }
@@ -1043,12 +1054,21 @@ int refresh(tmg_solver *h) {
     return 0;
 }
 
+// Host source of an upload of n doubles: the lent pinned staging buffer
+// (filled here) when there is one large enough, else hv itself. The staging
+// buffer is reused only after the stream has been synchronised.
+const double *host_src(tmg_solver *h, const double *hv, int64_t n) {
+    if (!h->stage || (size_t)n > h->stage_cap) return hv;
+    std::copy(hv, hv + n, h->stage);
+    return h->stage;
+}
+
 // host free-vector helpers
 int upload_free(tmg_solver *h, int l, const double *hv, double *d_full) {
     DevLevel &lv = h->lev[l];
     double *tmp = nullptr;
     TRY(dalloc(h, &tmp, lv.nfree));
-    HIPX(hipMemcpyAsync(tmp, hv, sizeof(double) * lv.nfree, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(tmp, host_src(h, hv, lv.nfree), sizeof(double) * lv.nfree, hipMemcpyHostToDevice, h->st));
     hipLaunchKernelGGL((k_scatter_full<double>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, lv.d_cidx, tmp,
                        d_full, nullptr);
     HIPX(hipStreamSynchronize(h->st));
@@ -1062,10 +1082,37 @@ int download_free(tmg_solver *h, int l, const double *d_full, double *hv) {
     TRY(dalloc(h, &tmp, lv.nfree));
     hipLaunchKernelGGL((k_gather<double>), dim3(nblk(lv.nfree)), dim3(TPB), 0, h->st, lv.nfree, lv.d_free, d_full, tmp,
                        nullptr);
-    HIPX(hipMemcpyAsync(hv, tmp, sizeof(double) * lv.nfree, hipMemcpyDeviceToHost, h->st));
+    const bool staged = h->stage && (size_t)lv.nfree <= h->stage_cap;
+    HIPX(hipMemcpyAsync(staged ? h->stage : hv, tmp, sizeof(double) * lv.nfree, hipMemcpyDeviceToHost, h->st));
     HIPX(hipStreamSynchronize(h->st));
+    if (staged) std::copy(h->stage, h->stage + lv.nfree, hv);
     dfree(h, tmp);
     return 0;
+}
+
+// Order-independent checksum (sum of the 64-bit words, modulo 2^64) of n
+// doubles in device memory, computed on the handle's stream into its second
+// partial-sum buffer; blocks until the value is on the host.
+int device_checksum(tmg_solver *h, const double *d, int64_t n, uint64_t *out) {
+    const unsigned nb = (unsigned)std::min<int64_t>(std::max<int64_t>(nblk(n), 1), std::min(h->npartial - 1, 1024));
+    uint64_t *part = reinterpret_cast<uint64_t *>(h->partial2);
+    hipLaunchKernelGGL(k_checksum_partial, dim3(nb), dim3(TPB), 0, h->st, n, reinterpret_cast<const uint64_t *>(d), part);
+    hipLaunchKernelGGL(k_checksum_final, dim3(1), dim3(TPB), 0, h->st, (int)nb, part);
+    uint64_t v = 0;
+    HIPX(hipMemcpyAsync(&v, part + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, h->st));
+    HIPX(hipStreamSynchronize(h->st));
+    *out = v;
+    return 0;
+}
+
+uint64_t host_checksum(const double *v, int64_t n) {
+    uint64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t w;
+        std::memcpy(&w, v + i, sizeof(w));
+        s += w;
+    }
+    return s;
 }
 
 int set_err(tmg_solver *h, int rc, const std::string &m) {
@@ -1087,6 +1134,8 @@ struct ExtRes {
     rocblas_handle rb = nullptr;
     PcgScalars *h_sc = nullptr;
     hipMemPool_t pool = nullptr;
+    double *stage = nullptr;
+    size_t stage_cap = 0;
 };
 
 int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, const ExtRes *ext, tmg_solver **out,
@@ -1154,6 +1203,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->rb = ext->rb;
         h->h_sc = ext->h_sc;
         h->pool = ext->pool;
+        h->stage = ext->stage;
+        h->stage_cap = ext->stage_cap;
         h->own_stream = h->own_rb = h->own_hsc = false;
     }
     auto fail = [&](int rc) {
@@ -1472,7 +1523,7 @@ int tmg_update_density(tmg_solver *h, const double *x) {
     const int64_t n = h->lev[0].ne * h->nsub;
     for (int64_t e = 0; e < n; ++e)
         if (!(x[e] >= 0.0) || !std::isfinite(x[e])) return set_err(h, TMG_ERR_ARG, "densities must be finite and >= 0");
-    HIPX(hipMemcpyAsync(h->xdens, x, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(h->xdens, host_src(h, x, n), sizeof(double) * n, hipMemcpyHostToDevice, h->st));
     return refresh(h);
 }
 
@@ -1692,6 +1743,8 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     *avg_ms = (double)ms / nrep;
     return TMG_OK;
 }
+
+int64_t tmg_device_bytes(const tmg_solver *h) { return h ? (int64_t)h->bytes : 0; }
 
 int tmg_solve_stats(const tmg_solver *h, int *escalated_at, int *escalations) {
     if (!h) return TMG_ERR_ARG;

@@ -177,6 +177,10 @@ class Solver:
         self._chk(lib().tmg_solve_stats(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def device_bytes(self):
+        """Device bytes the handle holds (allocations rounded up to 256 bytes)."""
+        return int(lib().tmg_device_bytes(self.h))
+
     def time_kernel(self, which, nrep=20):
         """Average launch time (ms) of a solver kernel, HIP events on the handle's stream."""
         ms = C.c_double()

@@ -85,10 +85,10 @@ class Server:
         self._meta.pop(rid, None)
 
     def stats(self):
-        s = (C.c_double * 6)()
+        s = (C.c_double * 7)()
         lib().tmg_server_stats(self.h, s)
         return {"bytes_reserved": s[0], "peak_bytes_reserved": s[1], "queued": int(s[2]), "running": int(s[3]),
-                "done": int(s[4]), "clock": s[5]}
+                "done": int(s[4]), "clock": s[5], "peak_running": int(s[6])}
 
     def clock(self):
         return lib().tmg_server_clock(self.h)

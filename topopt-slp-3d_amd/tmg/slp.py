@@ -59,6 +59,34 @@ def lp_volume(c, a, sL, sU, r, equality=False):
     return s, lam.value, rc != TMG_NOT_CONVERGED
 
 
+def feasibility_step(a, b, sL2, sU2, equality=False):
+    """The step of the fallback LP the reference solves when the trust-region
+    LP is infeasible (Implementation/StrSLP.m:271-285): minimise t1 + t2 over
+    s in [sL2, sU2], t >= 0, subject to a's - t1 + t2 <= -b (= -b for the
+    equality row). The optimum is the box point of least violation: the
+    smallest a's when even that exceeds -b, and for the equality row the
+    largest a's when even that falls short of -b. Returns (s, lam) with lam
+    the row multiplier in linprog's sign convention (lambda.ineqlin >= 0;
+    lambda.eqlin = +1 when the row is met from above, -1 from below).
+    Coordinates with a = 0 take the zero step. When the box meets the row
+    (no violation), the least-violation point is not unique; the step nearest
+    to s = 0 along the row is returned with lam = 0."""
+    a, sL2, sU2 = (np.asarray(v, dtype=np.float64) for v in (a, sL2, sU2))
+    s_lo = np.where(a > 0, sL2, np.where(a < 0, sU2, 0.0))     # argmin a's
+    s_hi = np.where(a > 0, sU2, np.where(a < 0, sL2, 0.0))     # argmax a's
+    lo, hi = float(a @ s_lo), float(a @ s_hi)
+    if lo > -b:
+        return s_lo, 1.0
+    if equality and hi < -b:
+        return s_hi, -1.0
+    # the row can be met inside the box: move from s = 0 toward the row
+    # (scaled extreme step), which meets it exactly
+    if (not equality and b <= 0) or b == 0:
+        return np.zeros_like(a), 0.0
+    t = -b / lo if b > 0 else -b / hi
+    return t * (s_lo if b > 0 else s_hi), 0.0
+
+
 @dataclass
 class SLPResult:
     xstar: np.ndarray          # the last filtered densities computed (StrSLP.m:584)
@@ -168,8 +196,7 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
             sU2 = np.minimum(0.8 * delta, 1.0 - x)
             sL2[fixed] = 0.0
             sU2[fixed] = 0.0
-            s = np.where(A > 0, sL2, np.where(A < 0, sU2, 0.0))
-            lam = 1.0
+            s, lam = feasibility_step(A, b, sL2, sU2, equality=not volineq)
         fval = float(Fgrad @ s)
         tm["LP"] += time.perf_counter() - t
 
