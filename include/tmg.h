@@ -155,6 +155,19 @@ int tmg_solve_stats(const tmg_solver *h, int *escalated_at, int *escalations);
 int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens,
              const tmg_filter *filt, double *Fgrad);
 
+/* tmg_grad without device allocation. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * kernels and results as tmg_grad; its device buffers (displacement,
+ * densities, g0, filtered gradient, mask) live in one 96 MiB array per GPU
+ * that the library holds until the process ends. Returns TMG_ERR_ARG when
+ * they need more (3*nnodes + 3*n_el*nsub doubles + n_el bytes, each section
+ * rounded to 256 bytes; the 192x96x96 cantilever needs 88 MB). Every call on
+ * a GPU uses that GPU's array: calls that overlap on one GPU (from several
+ * threads or handles) race; nothing in it needs to start at zero. */
+int tmg_grad_noalloc(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens,
+                     const tmg_filter *filt, double *Fgrad);
+
 /* Density filter [W,w] = ElemNeighborhood(str,rmin,opfilter)
  * (Implementation/ElemNeighborhood.m:1-100) for opfilter 1 or 2 on a
  * structured grid, fixed: optional 0/1 element mask (str.fixedDens).

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -141,6 +142,7 @@ struct tmg_solver {
     // fence writes back and invalidates its XCD's L2), so off by default
     bool fin_fused = false;
     hipStream_t st = nullptr;
+    int dev = 0;                      // device current at set-up (the stream's)
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
     // externally owned resources (a server slot): device memory comes from a
     // stream-ordered pool, and the stream, rocBLAS handle and pinned scalar
@@ -1232,6 +1234,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
             mask[fd - 1] = 1;
         }
     }
+    (void)hipGetDevice(&h->dev);
     if (!h->st && hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
         h->err = "hipStreamCreate failed";
         return fail(TMG_ERR_HIP);
@@ -1916,36 +1919,131 @@ void tmg_filter_destroy(tmg_filter *f) {
     delete f;
 }
 
-int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens, const tmg_filter *filt,
-             double *Fgrad) {
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Compliance gradient. tmg_grad allocates its device buffers per call;
+// tmg_grad_noalloc runs the same kernels on a file-scope device array instead,
+// for callers that cannot allocate device memory.
+// ---------------------------------------------------------------------------
+// Largest gradient tmg_grad_noalloc takes: displacement (3 nnodes doubles),
+// densities, g0 and the filtered gradient (n_el * nsub doubles each) and the
+// free-element mask (n_el bytes), each section 256-byte aligned, in 96 MiB
+// (enough for the 192x96x96 cantilever: 88 MB).
+constexpr int64_t GRAD_SCRATCH_DOUBLES = (int64_t)12 << 20;
+static __device__ __align__(256) double scratch[GRAD_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of device dev's copy of scratch, looked up on the first
+// call for that device with the device current (the array's name is not a
+// device address on the host). Returns nullptr with err set on failure.
+double *scratch_on_device(int dev, std::string &err) {
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        err = "device index out of range for the gradient scratch";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) {
+            err = "hipGetDevice failed";
+            return nullptr;
+        }
+        if (cur != dev && hipSetDevice(dev) != hipSuccess) {
+            err = "hipSetDevice failed";
+            return nullptr;
+        }
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+inline int64_t round32(int64_t n) { return (n + 31) & ~(int64_t)31; }   // 256-byte sections of doubles
+
+// g0 on the device (single resolution or multiresolution) and, with a filter,
+// Fgrad = W'(g0 ./ w); the result is copied to the host Fgrad. Device buffers:
+// du (ndof), dxe, dg, dF (nd each), dfd (n_el bytes, only with freedens).
+int grad_impl(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens, const tmg_filter *filt,
+              double *Fgrad, double *du, double *dxe, double *dg, double *dF, uint8_t *dfd) {
+    const DevLevel &l0 = h->lev[0];
+    const int64_t nd = l0.ne * h->nsub;
+    HIPX(hipMemcpyAsync(du, u, sizeof(double) * l0.ndof, hipMemcpyHostToDevice, h->st));
+    HIPX(hipMemcpyAsync(dxe, x, sizeof(double) * nd, hipMemcpyHostToDevice, h->st));
+    if (freedens) HIPX(hipMemcpyAsync(dfd, freedens, l0.ne, hipMemcpyHostToDevice, h->st));
+    if (h->nsub == 1)
+        hipLaunchKernelGGL(k_grad0, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe,
+                           freedens ? dfd : nullptr, h->K0_64, h->penal, h->E0, h->emin, dg);
+    else
+        hipLaunchKernelGGL(k_grad0_mr, dim3(nblk(nd)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe,
+                           freedens ? dfd : nullptr, h->K0_64, h->penal, h->E0, h->emin, dg);
+    int rc = 0;
+    if (filt) rc = filter_apply_dev(const_cast<tmg_filter *>(filt), dg, 1, dF, h->st);
+    if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * nd, hipMemcpyDeviceToHost, h->st));
+    HIPX(hipStreamSynchronize(h->st));
+    return rc ? set_err(h, rc, "grad kernel launch failed") : TMG_OK;
+}
+
+int grad_args(tmg_solver *h, const double *u, const double *x, const tmg_filter *filt, const double *Fgrad) {
     if (!h || !u || !x || !Fgrad) return TMG_ERR_ARG;
     const DevLevel &l0 = h->lev[0];
     const int m = h->mr;  // gradient lives on the density mesh (m = 2 with multiresolution)
     if (filt && (filt->nelx != m * l0.nelx || filt->nely != m * l0.nely || filt->nelz != m * l0.nelz))
         return set_err(h, TMG_ERR_ARG, "filter grid does not match the density grid");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tmg_grad(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens, const tmg_filter *filt,
+             double *Fgrad) {
+    TRY(grad_args(h, u, x, filt, Fgrad));
+    const DevLevel &l0 = h->lev[0];
     const int64_t nd = l0.ne * h->nsub;
     double *du = nullptr, *dxe = nullptr, *dg = nullptr, *dF = nullptr;
     uint8_t *dfd = nullptr;
-    TRY(dalloc(h, &du, l0.ndof));
-    TRY(dalloc(h, &dxe, nd));
-    TRY(dalloc(h, &dg, nd));
-    TRY(dalloc(h, &dF, nd));
-    if (freedens) TRY(dalloc(h, &dfd, l0.ne));
-    HIPX(hipMemcpyAsync(du, u, sizeof(double) * l0.ndof, hipMemcpyHostToDevice, h->st));
-    HIPX(hipMemcpyAsync(dxe, x, sizeof(double) * nd, hipMemcpyHostToDevice, h->st));
-    if (freedens) HIPX(hipMemcpyAsync(dfd, freedens, l0.ne, hipMemcpyHostToDevice, h->st));
-    if (h->nsub == 1)
-        hipLaunchKernelGGL(k_grad0, dim3(nblk(l0.ne)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
-                           h->K0_64, h->penal, h->E0, h->emin, dg);
-    else
-        hipLaunchKernelGGL(k_grad0_mr, dim3(nblk(nd)), dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz, du, dxe, dfd,
-                           h->K0_64, h->penal, h->E0, h->emin, dg);
-    int rc = 0;
-    if (filt) rc = filter_apply_dev(const_cast<tmg_filter *>(filt), dg, 1, dF, h->st);
-    if (!rc) HIPX(hipMemcpyAsync(Fgrad, filt ? dF : dg, sizeof(double) * nd, hipMemcpyDeviceToHost, h->st));
-    HIPX(hipStreamSynchronize(h->st));
+    int rc = dalloc(h, &du, l0.ndof);
+    if (!rc) rc = dalloc(h, &dxe, nd);
+    if (!rc) rc = dalloc(h, &dg, nd);
+    if (!rc) rc = dalloc(h, &dF, nd);
+    if (!rc && freedens) rc = dalloc(h, &dfd, l0.ne);
+    if (!rc) rc = grad_impl(h, u, x, freedens, filt, Fgrad, du, dxe, dg, dF, dfd);
+    (void)hipStreamSynchronize(h->st);
     dfree(h, du); dfree(h, dxe); dfree(h, dg); dfree(h, dF); dfree(h, dfd);
-    return rc ? set_err(h, rc, "grad kernel launch failed") : TMG_OK;
+    return rc;
+}
+
+int tmg_grad_noalloc(tmg_solver *h, const double *u, const double *x, const uint8_t *freedens,
+                     const tmg_filter *filt, double *Fgrad) {
+    TRY(grad_args(h, u, x, filt, Fgrad));
+    const DevLevel &l0 = h->lev[0];
+    const int64_t nd = l0.ne * h->nsub;
+    const int64_t need = round32(l0.ndof) + 3 * round32(nd) + round32((l0.ne + 7) / 8);
+    if (need > GRAD_SCRATCH_DOUBLES)
+        return set_err(h, TMG_ERR_ARG, "gradient of " + std::to_string(need * 8) + " bytes exceeds the " +
+                                           std::to_string(GRAD_SCRATCH_DOUBLES * 8) +
+                                           "-byte scratch of tmg_grad_noalloc (use tmg_grad)");
+    double *base = scratch_on_device(h->dev, h->err);
+    if (!base) return TMG_ERR_HIP;
+    double *du = base;
+    double *dxe = du + round32(l0.ndof);
+    double *dg = dxe + round32(nd);
+    double *dF = dg + round32(nd);
+    uint8_t *dfd = reinterpret_cast<uint8_t *>(dF + round32(nd));
+    return grad_impl(h, u, x, freedens, filt, Fgrad, du, dxe, dg, dF, dfd);
 }
 
 }  // extern "C"

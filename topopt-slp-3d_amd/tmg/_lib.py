@@ -76,6 +76,7 @@ _SIGS = {
     "tmg_ngrids": (_i, [_p]),
     "tmg_solve_stats": (_i, [_p, _pi, _pi]),
     "tmg_grad": (_i, [_p, _pd, _pd, _pu8, _p, _pd]),
+    "tmg_grad_noalloc": (_i, [_p, _pd, _pd, _pu8, _p, _pd]),
     "tmg_filter_create": (_i, [_i, _i, _i, _d, _d, _d, _d, _i, _pu8, C.POINTER(_p)]),
     "tmg_filter_apply": (_i, [_p, _pd, _i, _pd]),
     "tmg_filter_rowsum": (_i, [_p, _pd]),
