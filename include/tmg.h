@@ -200,8 +200,9 @@ int tmg_lp_volume(int64_t n, const double *c, const double *a, const double *sL,
  * (preconditioner application), 4 = fine-level residual with the first
  * Jacobi sweep folded in, 5 = fine-level post-sweep with the prolongation
  * folded in (both in the preconditioner precision), 6 = FP64 fine-level
- * apply with the PCG direction update folded in. Requires a prior
- * tmg_update_density. */
+ * apply with the PCG direction update folded in, 7 and 8 = the fine-level
+ * residual sweep and post-sweep on the FP64 PCG residual (the V-cycle's own
+ * forms, preconditioner precision). Requires a prior tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
 /* Instrumentation (no counterpart in the reference): device bytes the handle

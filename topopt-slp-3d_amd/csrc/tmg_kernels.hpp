@@ -890,6 +890,307 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
 }
 
 // ---------------------------------------------------------------------------
+// The spectral element apply with the node plane staged through LDS
+// (k_ebe_lds; same arithmetic as k_ebe_wht, SRC_X sources). Each thread loads
+// only its own node of a plane (threads 0..NQ-TPB-1 also one halo node of the
+// tile's (TX+1) x (TY+1)-node region) and writes it to LDS; the 2x2 patch of
+// an element column is then read from LDS instead of being loaded four times
+// from memory. All memory accesses are buffer loads and stores: per-thread
+// 32-bit offsets fixed for the launch plus one scalar plane offset per step
+// (no per-step 64-bit address arithmetic; out-of-range buffer accesses read 0
+// and are dropped, so every access stays inside its buffer). The next plane,
+// the next layer's moduli and the next output node's epilogue operands are
+// loaded one step ahead. One barrier per step: the corner exchange of plane L
+// and the staging of plane L + 2 share it (double-buffered). Requires every
+// vector to be smaller than 4 GiB (checked by the host). Jacobi-type
+// epilogues (JACOBI, POSTR) do not read the free-DOF mask: at fixed DOFs
+// invD = 0 and the source value is 0, so the sweep writes 0 there.
+// WPE: waves per SIMD the register allocation must allow (1 = no limit).
+// ---------------------------------------------------------------------------
+typedef unsigned int bu2 __attribute__((__vector_size__(8)));    // operand types of the buffer builtins
+typedef unsigned int bu4 __attribute__((__vector_size__(16)));
+typedef float f2v __attribute__((ext_vector_type(2)));            // element access goes through these
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef __amdgpu_buffer_rsrc_t brsrc_t;
+
+__device__ inline brsrc_t buf_rsrc(const void *p, uint32_t nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes, 0x00020000);
+}
+// three consecutive values at byte offset v + s
+__device__ inline void bld3(brsrc_t r, uint32_t v, uint32_t s, double (&o)[3]) {
+    const d2v a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v + 16, s, 0));
+}
+__device__ inline void bld3(brsrc_t r, uint32_t v, uint32_t s, float (&o)[3]) {
+    const f2v a = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v + 8, s, 0));
+}
+__device__ inline void bst3(brsrc_t r, uint32_t v, uint32_t s, const double (&o)[3]) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bu4, d2v{o[0], o[1]}), r, v, s, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, o[2]), r, v + 16, s, 0);
+}
+__device__ inline void bst3(brsrc_t r, uint32_t v, uint32_t s, const float (&o)[3]) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, f2v{o[0], o[1]}), r, v, s, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, o[2]), r, v + 8, s, 0);
+}
+__device__ inline double bld1(brsrc_t r, uint32_t v, uint32_t s, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+}
+__device__ inline float bld1(brsrc_t r, uint32_t v, uint32_t s, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+}
+
+constexpr int LDS_REGION_MAX = 300;   // (TX + 1)(TY + 1) for TY in {8, 16, 32}: at most 297
+
+template <typename T>
+struct EpiOps {
+    unsigned m[3];
+    T b[3], inv[3];
+    double r[3];
+};
+
+template <typename T, int EPI, int NSUB = 1, int WPE = 1>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_ebe_lds(int nelx, int nely, int nelz, int zchunk, int tyl, int ntx,
+                                                 const T *__restrict__ x_, const T *__restrict__ E_,
+                                                 const uint8_t *__restrict__ mask_, const T *__restrict__ b_,
+                                                 const T *__restrict__ invD_, const double *__restrict__ r_,
+                                                 T *__restrict__ out_, double *__restrict__ partial_,
+                                                 const int *__restrict__ done_, const T *__restrict__ kw_, int xcd_) {
+    if (done_ && *done_) return;
+    const int nx = nelx + 1, ny = nely + 1, nz = nelz + 1;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const int TY = 1 << tyl, TX = TPB >> tyl, RY = TY + 1, NQ = (TX + 1) * RY;
+    const int t = (int)threadIdx.x;
+    const int tx = t >> tyl, ty = t & (TY - 1);
+    int bt = (int)blockIdx.x, bz = (int)blockIdx.y;
+    if (xcd_) {
+        const int64_t lb = xcd_block((int64_t)blockIdx.y * gridDim.x + blockIdx.x, (int64_t)gridDim.x * gridDim.y);
+        bz = (int)(lb / gridDim.x);
+        bt = (int)(lb - (int64_t)bz * gridDim.x);
+    }
+    const int bxx = bt % ntx, byy = bt / ntx;
+    const int ix0 = bxx * (TX - 1) - 1, iy0 = byy * (TY - 1) - 1;   // node origin of the tile's region
+    const int ix = ix0 + tx, iy = iy0 + ty;                         // element column = its (0,0) node
+    const bool out_node = tx >= 1 && ty >= 1 && ix < nx && iy < ny;
+    const bool ecol = ix >= 0 && ix < nelx && iy >= 0 && iy < nely;
+    const int z0 = bz * zchunk, z1 = min(z0 + zchunk, nz);
+    // region nodes this thread stages: q = t and q = t + TPB, clamped into the grid
+    auto qoff = [&](int q) {
+        const int a = q / RY, bq = q - a * RY;
+        const int gx = min(max(ix0 + a, 0), nx - 1), gy = min(max(iy0 + bq, 0), ny - 1);
+        return (uint32_t)(gx * ny + gy) * 3u * (uint32_t)sizeof(T);
+    };
+    const uint32_t vq0 = qoff(t);
+    const bool has_q1 = t + TPB < NQ;
+    const uint32_t vq1 = has_q1 ? qoff(t + TPB) : 0u;
+    const uint32_t own = (uint32_t)(min(max(ix, 0), nx - 1) * ny + min(max(iy, 0), ny - 1));
+    const uint32_t pT = plane * 3u * (uint32_t)sizeof(T);   // bytes of one node plane of a T vector
+    const uint32_t nbT = pT * (uint32_t)nz;
+    const brsrc_t rx = buf_rsrc(x_, nbT);
+    const brsrc_t rE = buf_rsrc(E_, (uint32_t)nelx * nely * nelz * NSUB * (uint32_t)sizeof(T));
+    const brsrc_t rm = buf_rsrc(mask_, plane * 3u * (uint32_t)nz);
+    const brsrc_t rb = buf_rsrc(b_, b_ ? nbT : 0u);
+    const brsrc_t ri = buf_rsrc(invD_, invD_ ? nbT : 0u);
+    const brsrc_t rr = buf_rsrc(r_, r_ ? plane * 24u * (uint32_t)nz : 0u);
+    const brsrc_t ro = buf_rsrc(out_, nbT);
+    const uint32_t vE = (ecol ? (uint32_t)(ix * nely + iy) : 0u) * NSUB * (uint32_t)sizeof(T);
+    const uint32_t sEz = (uint32_t)nelx * nely * NSUB * (uint32_t)sizeof(T);
+
+    __shared__ T lin[2][3][LDS_REGION_MAX];
+    __shared__ T xch[2][3][3][TPB];
+
+    T R0[3], R1[3];
+    auto load_raw = [&](int zz) {
+        const uint32_t s = (uint32_t)min(max(zz, 0), nz - 1) * pT;
+        bld3(rx, vq0, s, R0);
+        if (has_q1) bld3(rx, vq1, s, R1);
+    };
+    auto store_lin = [&](int p) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lin[p][c][t] = R0[c];
+        if (has_q1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) lin[p][c][t + TPB] = R1[c];
+        }
+    };
+    auto patch_transform = [&](int p, T (&P)[3][4], T (&w)[3]) {
+        const int q = tx * RY + ty;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T v[4] = {lin[p][c][q], lin[p][c][q + 1], lin[p][c][q + RY], lin[p][c][q + RY + 1]};
+            wht4(v, P[c]);
+            w[c] = v[0];
+        }
+    };
+    auto load_E = [&](int ez, T (&dst)[NSUB]) {
+        const bool ok = ecol && ez >= 0 && ez < nelz;
+        const uint32_t s = (uint32_t)min(max(ez, 0), nelz - 1) * sEz;
+#pragma unroll
+        for (int q = 0; q < NSUB; ++q) {
+            const T v = bld1(rE, vE + q * (uint32_t)sizeof(T), s, T(0));
+            dst[q] = ok ? v : T(0);
+        }
+    };
+    auto load_epi = [&](int L, EpiOps<T> &e) {
+        const uint32_t zl = (uint32_t)min(L, nz - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            e.m[c] = (EPI == EPI_JACOBI || EPI == EPI_POSTR) ? 1u
+                     : __builtin_amdgcn_raw_buffer_load_b8(rm, own * 3u + c, zl * plane * 3u, 0);
+        if (EPI == EPI_RESID || EPI == EPI_JACOBI) bld3(rb, own * 3u * (uint32_t)sizeof(T), zl * pT, e.b);
+        if (EPI == EPI_JACOBI || EPI == EPI_POSTR) bld3(ri, own * 3u * (uint32_t)sizeof(T), zl * pT, e.inv);
+        if (EPI == EPI_RESIDR || EPI == EPI_POSTR) bld3(rr, own * 24u, zl * plane * 24u, e.r);
+    };
+
+    typedef const __attribute__((address_space(4))) T *cptr;
+    cptr kp = (cptr)kw_;
+    T Qtp[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Qtp[c][k] = T(0);
+    double dacc = 0.0;
+
+    auto step = [&](const int L, const int p, T (&Pb)[3][4], T (&Pt)[3][4], T (&wb)[3], T (&wt)[3], T (&Ee)[NSUB],
+                    T (&En)[NSUB], EpiOps<T> &ec, EpiOps<T> &en) {
+        patch_transform(p, Pt, wt);   // plane L + 1
+        load_E(L + 1, En);
+        T xh[3][8];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xh[d][k] = Pb[d][k] + Pt[d][k];
+                xh[d][k + 4] = Pb[d][k] - Pt[d][k];
+            }
+        T yh[3][8];
+        if constexpr (NSUB == 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (c > 0) asm volatile("" : "+s"(kp) : "v"(yh[c - 1][7]));
+                yh[c][0] = T(0);
+#pragma unroll
+                for (int k = 1; k < 8; ++k) {
+                    T acc = T(0);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if ((WHT_NZ[c][d] >> k) & 1)
+                            acc = fma(kp[wht_slot(c, d, k)], xh[d][k ^ WHT_BIT[c] ^ WHT_BIT[d]], acc);
+                    yh[c][k] = Ee[0] * acc;
+                }
+            }
+        } else {
+            T Eh[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Eh[q] = Ee[q];
+#pragma unroll
+            for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (!(q & bit)) {
+                        const T u = Eh[q], v = Eh[q | bit];
+                        Eh[q] = u + v;
+                        Eh[q | bit] = u - v;
+                    }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                yh[c][0] = T(0);
+#pragma unroll
+                for (int k = 1; k < 8; ++k) {
+                    asm volatile("" : "+s"(kp) : "v"(k > 1 ? yh[c][k - 1] : (c > 0 ? yh[c - 1][7] : Eh[0])));
+                    T grp[8];
+                    bool has[8] = {false, false, false, false, false, false, false, false};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int l = 1; l < 8; ++l)
+                            if ((WHT_MR_NZ[8 * c + k] >> (8 * d + l)) & 1) {
+                                const int m = k ^ l ^ WHT_BIT[c] ^ WHT_BIT[d];
+                                const T coef = kp[wht_mr_slot(c, k, d, l)];
+                                grp[m] = has[m] ? fma(coef, xh[d][l], grp[m]) : coef * xh[d][l];
+                                has[m] = true;
+                            }
+                    T acc = T(0);
+#pragma unroll
+                    for (int m = 0; m < 8; ++m)
+                        if (has[m]) acc = fma(Eh[m], grp[m], acc);
+                    yh[c][k] = acc;
+                }
+            }
+        }
+        asm volatile("" : "+s"(kp) : "v"(yh[2][7]));
+        T cv[3][4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T S[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                S[k] = Qtp[c][k] + (yh[c][k] + yh[c][k + 4]);
+                Qtp[c][k] = yh[c][k] - yh[c][k + 4];
+            }
+            wht4(S, cv[c]);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xch[p][j][c][t] = cv[c][j + 1];
+        store_lin(p ^ 1);          // plane L + 2
+        load_raw(L + 3);
+        load_epi(L + 1, en);
+        __syncthreads();
+        if (L >= z0 && out_node) {
+            T o[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T y = cv[c][0] + xch[p][0][c][t - 1] + xch[p][1][c][t - TY] + xch[p][2][c][t - TY - 1];
+                const bool m = ec.m[c] != 0;
+                const T w = wb[c];
+                if (EPI == EPI_PLAIN) {
+                    o[c] = m ? y : T(0);
+                } else if (EPI == EPI_RESID) {
+                    o[c] = m ? (ec.b[c] - y) : T(0);
+                } else if (EPI == EPI_JACOBI) {
+                    o[c] = m ? (w + ec.inv[c] * (ec.b[c] - y)) : T(0);
+                } else if (EPI == EPI_DOT) {
+                    o[c] = m ? y : T(0);
+                    dacc += (double)w * (double)o[c];
+                } else if (EPI == EPI_RESIDR) {
+                    o[c] = m ? ((T)ec.r[c] - y) : T(0);
+                } else {  // EPI_POSTR
+                    o[c] = m ? (w + ec.inv[c] * ((T)ec.r[c] - y)) : T(0);
+                    dacc += ec.r[c] * (double)o[c];
+                }
+            }
+            bst3(ro, own * 3u * (uint32_t)sizeof(T), (uint32_t)L * pT, o);
+        }
+    };
+
+    T PA[3][4], PB[3][4], wA[3], wB[3], EA[NSUB], EB[NSUB];
+    EpiOps<T> epA, epB;
+    load_raw(z0 - 1);
+    store_lin(0);
+    load_raw(z0);
+    load_E(z0 - 1, EA);
+    __syncthreads();
+    patch_transform(0, PA, wA);
+    store_lin(1);
+    load_raw(z0 + 1);
+    __syncthreads();
+    int L = z0 - 1;
+    for (; L + 1 < z1; L += 2) {
+        step(L, 1, PA, PB, wA, wB, EA, EB, epA, epB);
+        step(L + 1, 0, PB, PA, wB, wA, EB, EA, epB, epA);
+    }
+    if (L < z1) step(L, 1, PA, PB, wA, wB, EA, EB, epA, epB);
+    if (EPI == EPI_DOT || EPI == EPI_POSTR) block_reduce_store(dacc, partial_, (int64_t)bz * gridDim.x + bt);
+}
+
+
+// ---------------------------------------------------------------------------
 // Coarse-level operator: assembled 27-point block stencil of the Galerkin
 // operator P'AP, stored symmetrically: only the centre block and the 13
 // "forward" neighbour blocks (offset index k = (dz+1)*9 + (dx+1)*3 + (dy+1)

@@ -124,6 +124,7 @@ struct tmg_solver {
     bool wht = false;
     bool wht_pf64 = true;             // spectral apply with the next plane prefetched: FP64 (faster)
     bool wht_pf32 = false;            // and FP32 (slower: the extra registers cost a wave)
+    bool ebe_lds = true;              // SRC_X applies of the spectral form with the node plane staged through LDS (k_ebe_lds)
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
     KWht<float> kw32{};
@@ -362,6 +363,16 @@ void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
 #define WHT_ARGS a.nelx, a.nely, a.nelz, a.zchunk, h->wht_tyl, h->wht_ntx, a.x, a.E, a.mask, a.b, a.invD, a.r, a.ec, \
                  a.cnx, a.cny, a.s, a.dold, a.dnew, a.sc, a.out, a.partial, a.done, KWof<T>(h), h->xcd_wht
         const bool pf = std::is_same<T, double>::value ? h->wht_pf64 : h->wht_pf32;
+        if constexpr (SRC == SRC_X && EPI != EPI_DIRDOT) {
+            if (h->ebe_lds && h->nsub == 1) {
+                // FP64: registers for 3 waves per SIMD; FP32 Jacobi sweeps: 5
+                constexpr int WPE = std::is_same<T, double>::value ? 3 : (EPI == EPI_JACOBI ? 5 : 1);
+                hipLaunchKernelGGL((k_ebe_lds<T, EPI, 1, WPE>), gw, dim3(TPB), 0, h->st, a.nelx, a.nely, a.nelz, a.zchunk,
+                                   h->wht_tyl, h->wht_ntx, a.x, a.E, a.mask, a.b, a.invD, a.r, a.out, a.partial, a.done,
+                                   KWof<T>(h), h->xcd_wht);
+                return;
+            }
+        }
         if (h->nsub == 8) {
             // multiresolution: one form per precision (FP64 prefetching)
             if constexpr (SRC == SRC_X) {
@@ -442,6 +453,14 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
         const int zc = std::is_same<T, double>::value ? h->l1_zc64 : h->l1_zc32;
         constexpr bool PF8 = std::is_same<T, double>::value;
         dim3 gw((unsigned)(h->l1_ntx * h->l1_nty), (unsigned)((lv.nz + zc - 1) / zc));
+        if constexpr (std::is_same<T, double>::value && EPI != EPI_DIRDOT && EPI != EPI_RESIDR) {
+            if (h->ebe_lds) {   // measured faster in FP64 only (FP32: the extra registers cost a wave)
+                hipLaunchKernelGGL((k_ebe_lds<T, EPI, 8, 1>), gw, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, zc,
+                                   h->l1_tyl, h->l1_ntx, x, E1of<T>(h), lv.mask, b, invof<T>(lv),
+                                   (const double *)nullptr, out, partial, done, KWGof<T>(h), h->xcd_wht);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_ebe_wht<T, T, SRC_X, EPI, PF8, 8>), gw, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, zc,
                            h->l1_tyl, h->l1_ntx, x, E1of<T>(h), lv.mask, b, invof<T>(lv), (const double *)nullptr,
                            (const T *)nullptr, 1, 1, (const T *)nullptr, (const double *)nullptr, (double *)nullptr,
@@ -1195,6 +1214,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_FUSED_MAXNODES")) h->fused_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_ST3_MAXNODES")) h->st3_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
+    if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
     if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
@@ -1399,6 +1419,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->l1_wht = wht_mr_table(K8.data(), h->kwg64, h->kwg32);
         if (h->l1_wht) wht_tiles(h->lev[1].nx, h->lev[1].ny, h->l1_tyl, h->l1_ntx, h->l1_nty);
     }
+    // the LDS-staged apply addresses its vectors through 32-bit buffer offsets
+    if ((uint64_t)h->lev[0].ndof * 8 >= ((uint64_t)1 << 32) - 4096) h->ebe_lds = false;
     h->coarse_symv = h->nc >= 16384;
     if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
     h->cs_nb = (int)((h->nc + CS_B - 1) / CS_B);
@@ -1452,6 +1474,10 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                 k64 = h->wht_pf64 ? reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, true>)
                                   : reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_DOT, false>);
             }
+            if (h->ebe_lds && h->nsub == 1) {
+                k32 = reinterpret_cast<const void *>(&k_ebe_lds<float, EPI_POSTR, 1, 1>);
+                k64 = reinterpret_cast<const void *>(&k_ebe_lds<double, EPI_DOT, 1, 3>);
+            }
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o32, k32, TPB, 0);
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o64, k64, TPB, 0);
             ncu = std::max(ncu, 1);
@@ -1462,7 +1488,9 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
                 (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
                     &p32, reinterpret_cast<const void *>(&k_ebe_wht<float, float, SRC_X, EPI_JACOBI, false, 8>), TPB, 0);
                 (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &p64, reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_JACOBI, true, 8>), TPB, 0);
+                    &p64, h->ebe_lds ? reinterpret_cast<const void *>(&k_ebe_lds<double, EPI_JACOBI, 8, 1>)
+                                     : reinterpret_cast<const void *>(&k_ebe_wht<double, double, SRC_X, EPI_JACOBI, true, 8>),
+                    TPB, 0);
                 const int64_t t1 = (int64_t)h->l1_ntx * h->l1_nty;
                 h->l1_zc32 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p32, 1));
                 h->l1_zc64 = wht_zchunk(t1, h->lev[1].nz, (int64_t)ncu * std::max(p64, 1));
@@ -1673,7 +1701,7 @@ int tmg_level_info(tmg_solver *h, int level, int *nel3, int64_t *nfree, int64_t 
 int tmg_ngrids(const tmg_solver *h) { return h ? h->L : 0; }
 
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
-    if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 6) return TMG_ERR_ARG;
+    if (!h || !avg_ms || nrep <= 0 || which < 0 || which > 8) return TMG_ERR_ARG;
     if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
     if (which == 2 && h->L < 3) return set_err(h, TMG_ERR_ARG, "level 2 is the coarsest level here");
     if (which >= 4 && h->nsub > 1)
@@ -1697,6 +1725,19 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
             DevLevel &lv = h->lev[l];
             if (mixed) apply<float, EPI_JACOBI>(h, l, (const float *)lv.x, (const float *)lv.b, (float *)lv.t, nullptr, nullptr);
             else apply<double, EPI_JACOBI>(h, l, (const double *)lv.x, (const double *)lv.b, (double *)lv.t, nullptr, nullptr);
+        } else if (which == 7 || which == 8) {
+            // the fine-level cycle's residual and post-sweep on the FP64 residual
+            if (mixed) {
+                EbeArgs<float, float> a = ebe_args<float, float>(h);
+                a.x = (const float *)l0.x; a.r = h->pr; a.out = (float *)l0.t; a.partial = h->partial;
+                if (which == 7) launch_ebe<float, float, SRC_X, EPI_RESIDR>(h, a);
+                else launch_ebe<float, float, SRC_X, EPI_POSTR>(h, a);
+            } else {
+                EbeArgs<double, double> a = ebe_args<double, double>(h);
+                a.x = (const double *)l0.x; a.r = h->pr; a.out = (double *)l0.t; a.partial = h->partial;
+                if (which == 7) launch_ebe<double, double, SRC_X, EPI_RESIDR>(h, a);
+                else launch_ebe<double, double, SRC_X, EPI_POSTR>(h, a);
+            }
         } else if (which == 3) {
             const double *s64;
             const float *s32;
