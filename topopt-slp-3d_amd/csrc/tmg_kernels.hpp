@@ -1558,6 +1558,47 @@ __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict
     else block_reduce_store(s, partial, blockIdx.x);
 }
 
+// The PCG's vector passes with the solution update deferred by one iteration
+// (x_{k+1} = x_k + alpha_k d_k is the same sum, done when d_k is read anyway):
+// k_r_update: r -= alpha q; partial ||r||^2; and the next preconditioner
+// cycle's first Jacobi sweep from zero, x0 = invD r (in its precision TX).
+template <typename TX>
+__global__ __launch_bounds__(TPB) void k_r_update(int64_t n, double *__restrict__ r, const double *__restrict__ q,
+                                                  const TX *__restrict__ invD, TX *__restrict__ x0, double *partial,
+                                                  PcgScalars *sc) {
+    if (sc->done) return;
+    const double al = sc->alpha;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        const double ri = r[i] - al * q[i];
+        r[i] = ri;
+        s += ri * ri;
+        x0[i] = invD[i] * (TX)ri;
+    }
+    block_reduce_store(s, partial, blockIdx.x);
+}
+
+// k_dir_xupdate: x += alpha d with the previous iteration's alpha and d (0
+// before the first iteration: alpha = 0 and d = 0 at the start of a solve),
+// then d = s + beta d (d = s on a restart iteration).
+template <typename TS>
+__global__ __launch_bounds__(TPB) void k_dir_xupdate(int64_t n, const TS *__restrict__ s, double *__restrict__ d,
+                                                     double *__restrict__ x, const PcgScalars *sc) {
+    if (sc->done) return;
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const double di = d[i];
+    x[i] = x[i] + sc->alpha * di;
+    d[i] = (sc->it == sc->restart) ? (double)s[i] : sc->beta * di + (double)s[i];
+}
+
+// The deferred update of the last iteration, once the PCG has stopped.
+__global__ __launch_bounds__(TPB) void k_x_final(int64_t n, double *__restrict__ x, const double *__restrict__ d,
+                                                 const PcgScalars *sc) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) x[i] = x[i] + sc->alpha * d[i];
+}
+
 // Single-block, fixed-order final reduction and the scalar PCG recurrences:
 // strided per-thread sums, then wave shuffles and the 4 wave sums in order
 // (one barrier).

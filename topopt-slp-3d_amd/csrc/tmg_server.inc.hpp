@@ -304,6 +304,7 @@ void srv_drive_loop(tmg_server *S) {
             tmg_solver *h = r->h;
             std::string msg = h->err;
             int rc = r->rc;
+            if (rc >= 0) rc = std::min(rc, solve_finish(h));
             if (rc >= 0) {
                 r->u.assign(h->lev[0].nfree, 0.0);
                 int d = download_free(h, 0, h->px, r->u.data());

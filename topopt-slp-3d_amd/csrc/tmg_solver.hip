@@ -142,6 +142,9 @@ struct tmg_solver {
     // a k_finalize launch: measured much slower (each block's device-scope
     // fence writes back and invalidates its XCD's L2), so off by default
     bool fin_fused = false;
+    // solution update deferred into the next direction update, and the
+    // cycle's first Jacobi sweep folded into the residual update
+    bool xdefer = true;
     hipStream_t st = nullptr;
     int dev = 0;                      // device current at set-up (the stream's)
     hipEvent_t ev = nullptr;          // completion marker of the last launched chunk
@@ -601,13 +604,16 @@ bool fused_ok(const tmg_solver *h) { return h->fused && h->nsub == 1 && h->mg.sm
 //   eH = coarse correction (P' rh); x += P eH
 //   s  = x + M \ (r - A x)          (post-sweep, partial[] += r . s)
 // Returns s; the partial sums come from the post-sweep's blocks.
+// x0_ready: the first sweep's x = M \ r is already in the level's x vector
+// (written by the PCG's residual update).
 template <typename T>
-T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done) {
+T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done, bool x0_ready = false) {
     DevLevel &l0 = h->lev[0];
     DevLevel &l1 = h->lev[1];
     T *xa = (T *)l0.x, *xb = (T *)l0.t;
-    hipLaunchKernelGGL((k_jacobi_zero<T, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, invof<T>(l0),
-                       xa, done);
+    if (!x0_ready)
+        hipLaunchKernelGGL((k_jacobi_zero<T, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r,
+                           invof<T>(l0), xa, done);
     EbeArgs<T, T> a = ebe_args<T, T>(h);
     a.x = xa; a.r = r; a.out = xb; a.done = done;
     launch_ebe<T, T, SRC_X, EPI_RESIDR>(h, a);
@@ -632,14 +638,14 @@ bool fine_r_ok(const tmg_solver *h) {
 // Returns s as a typed pointer through one of the two out-params; *nrho > 0
 // when partial[0..*nrho) already hold the blocks' sums of r . s.
 void precondition(tmg_solver *h, const double *r, const double **s64, const float **s32, const int *done,
-                  int *nrho = nullptr) {
+                  int *nrho = nullptr, bool x0_ready = false) {
     DevLevel &l0 = h->lev[0];
     *s64 = nullptr;
     *s32 = nullptr;
     if (nrho) *nrho = 0;
     if (!fused_ok(h) && fine_r_ok(h)) {
-        if (h->pprec == TMG_MIXED) *s32 = vcycle_fine_r<float>(h, r, done);
-        else *s64 = vcycle_fine_r<double>(h, r, done);
+        if (h->pprec == TMG_MIXED) *s32 = vcycle_fine_r<float>(h, r, done, x0_ready);
+        else *s64 = vcycle_fine_r<double>(h, r, done, x0_ready);
         if (nrho) *nrho = (int)apply_blocks(h, 0, h->pprec != TMG_MIXED);
         return;
     }
@@ -652,6 +658,20 @@ void precondition(tmg_solver *h, const double *r, const double **s64, const floa
     } else {
         *s64 = vcycle<double>(h, 0, r, done);
     }
+}
+
+// the PCG runs with the deferred solution update (k_dir_xupdate, k_r_update)
+bool xdefer_ok(const tmg_solver *h) { return h->xdefer && !fused_ok(h) && fine_r_ok(h) && !h->fin_fused; }
+
+// x0 = M \ r at the fine level in the running preconditioner precision
+void first_sweep_from_r(tmg_solver *h) {
+    DevLevel &l0 = h->lev[0];
+    if (h->pprec == TMG_MIXED)
+        hipLaunchKernelGGL((k_jacobi_zero<float, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, h->pr,
+                           l0.inv32, (float *)l0.x, (const int *)nullptr);
+    else
+        hipLaunchKernelGGL((k_jacobi_zero<double, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, h->pr,
+                           l0.inv64, (double *)l0.x, (const int *)nullptr);
 }
 
 // q = A d with d = beta d_old + s formed on the fly; writes d_new, partial d.q.
@@ -674,7 +694,8 @@ void pcg_iteration(tmg_solver *h, int parity) {
     const int napply = (int)apply_blocks(h, 0, true);
     const int finrr = h->fin_fused ? FIN_RR : FIN_NONE;
     int nrho = 0;
-    precondition(h, h->pr, &s64, &s32, done, &nrho);
+    const bool xd = xdefer_ok(h);
+    precondition(h, h->pr, &s64, &s32, done, &nrho, xd);
     if (fused_ok(h)) {
         // rho = r' s (partials from the post-smoothing apply, preconditioner precision)
         const int npost = (int)apply_blocks(h, 0, s32 == nullptr);
@@ -696,8 +717,14 @@ void pcg_iteration(tmg_solver *h, int parity) {
             else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
             hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
         }
-        if (s32) hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
-        else hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
+        if (xd) {
+            if (s32) hipLaunchKernelGGL((k_dir_xupdate<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->px, h->d_sc);
+            else hipLaunchKernelGGL((k_dir_xupdate<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->px, h->d_sc);
+        } else if (s32) {
+            hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
+        } else {
+            hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
+        }
         if (h->fin_fused && !h->fine_stencil) {
             // q = A d, d'q, alpha in the apply's last block
             EbeArgs<double, double> a = ebe_args<double, double>(h);
@@ -708,8 +735,18 @@ void pcg_iteration(tmg_solver *h, int parity) {
             hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr,
                                h->d_sc);
         }
-        hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial,
-                           h->d_sc, finrr);
+        if (xd) {
+            DevLevel &l0 = h->lev[0];
+            if (h->pprec == TMG_MIXED)
+                hipLaunchKernelGGL((k_r_update<float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv32,
+                                   (float *)l0.x, h->partial, h->d_sc);
+            else
+                hipLaunchKernelGGL((k_r_update<double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv64,
+                                   (double *)l0.x, h->partial, h->d_sc);
+        } else {
+            hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial,
+                               h->d_sc, finrr);
+        }
     }
     if (!h->fin_fused)
         hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
@@ -768,6 +805,7 @@ int solve_begin(tmg_solver *h) {
     hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pf, h->pf, h->partial2, nullptr);
     hipLaunchKernelGGL(k_finalize<FIN_INIT>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, (int)gdot,
                        h->partial2, h->d_sc);
+    if (xdefer_ok(h)) first_sweep_from_r(h);
     TRY(check_launch(h));
     if (h->use_graph && !h->graph_exec) {
         // capture GRAPH_ITERS consecutive iterations; iterations past
@@ -811,6 +849,7 @@ int solve_check(tmg_solver *h) {
     if (it - h->stall_it < h->stall_window) return 0;
     if (!(nr > h->stall_factor * h->stall_nr)) { h->stall_it = it; h->stall_nr = nr; return 0; }
     use_cycle_precision(h, TMG_FP64);
+    if (xdefer_ok(h)) first_sweep_from_r(h);   // the level vectors were cleared
     ++h->escalations;
     h->escalated_at = it;
     if (h->use_graph && !h->graph_exec64) {
@@ -845,6 +884,14 @@ int solve_result(tmg_solver *h, int *it, double *relres) {
     return (h->h_sc->nr > h->mg.tolMG) ? TMG_NOT_CONVERGED : TMG_OK;
 }
 
+// After the PCG has stopped: the last iteration's deferred solution update.
+int solve_finish(tmg_solver *h) {
+    if (!xdefer_ok(h)) return 0;
+    const int64_t n = h->lev[0].ndof;
+    hipLaunchKernelGGL(k_x_final, dim3(nblk(n)), dim3(TPB), 0, h->st, n, h->px, h->pd, h->d_sc);
+    return check_launch(h);
+}
+
 // Blocking device-resident solve on px (x0 in, x out) and pf.
 int solve_resident(tmg_solver *h, int *it, double *relres) {
     TRY(solve_begin(h));
@@ -856,6 +903,7 @@ int solve_resident(tmg_solver *h, int *it, double *relres) {
         TRY(solve_check(h));
         chunk = std::min(chunk * 2, 8);
     }
+    TRY(solve_finish(h));
     return solve_result(h, it, relres);
 }
 
@@ -1215,6 +1263,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_ST3_MAXNODES")) h->st3_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
+    if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
     if (const char *ev = getenv("TMG_EBE_UNTILED")) h->ebe_tiled = ev[0] == '0';
