@@ -60,12 +60,13 @@ typedef struct tmg_problem {
     int64_t nfree;
 } tmg_problem;
 
-/* The `mg` struct (Implementation/TopOpt3D.m:171-188). cycle = 0 (V) or 1
- * (W), smoother = 0 (Jacobi) and opchol = 0 (coarse Cholesky) are accepted;
- * the FMV cycle and the SOR/SSOR smoothers return TMG_ERR_UNSUPPORTED. */
+/* The `mg` struct (Implementation/TopOpt3D.m:171-188). cycle = 0 (V), 1 (W)
+ * or 2 (FMV), smoother = 0 (Jacobi) and opchol = 0 (coarse Cholesky) are
+ * accepted; the SOR/SSOR smoothers and the coarse PCG return
+ * TMG_ERR_UNSUPPORTED. */
 typedef struct tmg_mg_params {
     int ngrids;          /* mg.ngrids                                             */
-    int cycle;           /* mg.cycle        (0 = V, 1 = W)                        */
+    int cycle;           /* mg.cycle        (0 = V, 1 = W, 2 = FMV)               */
     int smoother;        /* mg.smoother     (0 = Jacobi)                          */
     double omega;        /* mg.omega                                              */
     int smoothiter1;     /* mg.smoothiter1                                        */
@@ -114,9 +115,10 @@ int tmg_solve(tmg_solver *h, const double *x0, const double *f, double *x_out, i
 int tmg_solve_device(tmg_solver *h, double *d_x, const double *d_f, int *it, double *relres);
 
 /* x = VCycle(A,b,zeros,niter1,niter2,ngrids,1,...) (Implementation/VCycle.m:1-45),
- * or WCycle(...) (Implementation/WCycle.m:1-27) for a handle set up with
- * mg->cycle = 1: one preconditioner cycle at the finest level on nfree host
- * vectors. */
+ * WCycle(...) (Implementation/WCycle.m:1-27) for a handle set up with
+ * mg->cycle = 1, or FMVCycle(A,b,niter1,niter2,ngrids,1,...)
+ * (Implementation/FMVCycle.m:1-41) with mg->cycle = 2: one preconditioner
+ * cycle at the finest level on nfree host vectors. */
 int tmg_vcycle(tmg_solver *h, const double *b, double *x);
 
 /* x = Smooth(A{level},M{level},L,U,b,x0,niter,0) (Implementation/Smooth.m:1-24),

@@ -54,6 +54,8 @@ def lib():
         L.or_solve_mg.restype = _i
         L.or_solve_mg.argtypes = [_p, _p, _p, _i, _i, _d, _i, _p]
         L.or_wcycle.argtypes = [_p, _i, _p, _p, _i, _i, _p]
+        L.or_vcycle_x0.argtypes = [_p, _i, _p, _p, _i, _i, _p]
+        L.or_fmvcycle.argtypes = [_p, _i, _p, _i, _i, _p]
         L.or_solve_mg_cycle.restype = _i
         L.or_solve_mg_cycle.argtypes = [_p, _p, _p, _i, _i, _i, _d, _i, _p]
         L.or_csr_nnz.restype = _i64
@@ -183,6 +185,19 @@ class Oracle:
         x = np.zeros_like(b)
         x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         lib().or_wcycle(self.h, level, _ptr(b), _ptr(x0a), nu1, nu2, _ptr(x))
+        return x
+
+    def vcycle_x0(self, b, x0, nu1=1, nu2=1, level=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        lib().or_vcycle_x0(self.h, level, _ptr(b), _ptr(x0a), nu1, nu2, _ptr(x))
+        return x
+
+    def fmvcycle(self, b, nu1=1, nu2=1, level=0):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        lib().or_fmvcycle(self.h, level, _ptr(b), nu1, nu2, _ptr(x))
         return x
 
     def solve(self, f, x0=None, nu1=1, nu2=1, tol=1e-8, maxiter=5000, cycle=0):

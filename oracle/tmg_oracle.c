@@ -665,25 +665,47 @@ static void coarse_solve(const or_ctx *c, const double *b, double *x) {
     free(y);
 }
 
-void or_vcycle(const or_ctx *c, int nv, const double *b, int nu1, int nu2, double *x) {
+/* x = VCycle(A, b, x0, nu1, nu2, ..., nv) (Implementation/VCycle.m:23-43);
+ * x0 may be NULL (zero initial guess, as SolveMG calls it). */
+void or_vcycle_x0(const or_ctx *c, int nv, const double *b, const double *x0, int nu1, int nu2, double *x) {
     int64_t n = c->lev[nv].nfree, nc = c->lev[nv + 1].nfree;
     double *zero = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     double *rh = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
     double *rH = (double *)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
     double *eH = (double *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
     double *eh = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-    or_smooth(c, nv, b, zero, nu1, x);                  /* VCycle.m:23 */
+    or_smooth(c, nv, b, x0 ? x0 : zero, nu1, x);       /* VCycle.m:23 */
     spmv(&c->A[nv], x, rh);                             /* VCycle.m:24 */
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) rh[i] = b[i] - rh[i];
     spmv(&c->PT[nv], rh, rH);                           /* VCycle.m:25 */
     if (nv == c->L - 2) coarse_solve(c, rH, eH);        /* VCycle.m:35-36 */
-    else or_vcycle(c, nv + 1, rH, nu1, nu2, eH);        /* VCycle.m:39 */
+    else or_vcycle_x0(c, nv + 1, rH, NULL, nu1, nu2, eH);   /* VCycle.m:39 */
     spmv(&c->P[nv], eH, eh);                            /* VCycle.m:41 */
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) x[i] = x[i] + eh[i];
     or_smooth(c, nv, b, x, nu2, x);                     /* VCycle.m:43 */
     free(zero); free(rh); free(rH); free(eH); free(eh);
+}
+
+void or_vcycle(const or_ctx *c, int nv, const double *b, int nu1, int nu2, double *x) {
+    or_vcycle_x0(c, nv, b, NULL, nu1, nu2, x);
+}
+
+/* x = FMVCycle(A, b, nu1, nu2, ..., nv) (Implementation/FMVCycle.m:21-39):
+ * restrict b, solve the coarsest system directly or recurse, prolong, then
+ * one V-cycle from that initial guess. */
+void or_fmvcycle(const or_ctx *c, int nv, const double *b, int nu1, int nu2, double *x) {
+    int64_t n = c->lev[nv].nfree, nc = c->lev[nv + 1].nfree;
+    double *rH = (double *)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    double *xc = (double *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+    double *xp = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    spmv(&c->PT[nv], b, rH);                            /* FMVCycle.m:21 */
+    if (nv == c->L - 2) coarse_solve(c, rH, xc);        /* FMVCycle.m:22-33 */
+    else or_fmvcycle(c, nv + 1, rH, nu1, nu2, xc);      /* FMVCycle.m:35 */
+    spmv(&c->P[nv], xc, xp);                            /* FMVCycle.m:37 */
+    or_vcycle_x0(c, nv, b, xp, nu1, nu2, x);            /* FMVCycle.m:38 */
+    free(rH); free(xc); free(xp);
 }
 
 /* x = WCycle(A, b, x0, nu1, nu2, ..., nv) (Implementation/WCycle.m:1-27):
@@ -738,6 +760,7 @@ int or_solve_mg_cycle(const or_ctx *c, double *x, const double *f, int cycle, in
     while (nr > tol && it <= maxiter) {                 /* SolveMG.m:30 */
         if (c->L >= 2) {                                /* SolveMG.m:31-37 */
             if (cycle == 1) or_wcycle(c, 0, r, NULL, nu1, nu2, s);
+            else if (cycle == 2) or_fmvcycle(c, 0, r, nu1, nu2, s);
             else or_vcycle(c, 0, r, nu1, nu2, s);
         } else {
             memcpy(s, r, bytes);

@@ -66,7 +66,11 @@ int or_solve_mg(const or_ctx *c, double *x, const double *f, int nu1, int nu2,
 /* x = WCycle(A, b, x0, nu1, nu2, ..., nv=level) (Implementation/WCycle.m:1-27);
  * x0 may be NULL for a zero initial guess. */
 void or_wcycle(const or_ctx *c, int level, const double *b, const double *x0, int nu1, int nu2, double *x);
-/* SolveMG with the cycle switch of Implementation/SolveMG.m:31-37 (0 V, 1 W). */
+/* x = VCycle(A, b, x0, nu1, nu2, ..., nv) from an initial guess (x0 NULL = zero). */
+void or_vcycle_x0(const or_ctx *c, int level, const double *b, const double *x0, int nu1, int nu2, double *x);
+/* x = FMVCycle(A, b, nu1, nu2, ..., nv) (Implementation/FMVCycle.m:1-41). */
+void or_fmvcycle(const or_ctx *c, int level, const double *b, int nu1, int nu2, double *x);
+/* SolveMG with the cycle switch of Implementation/SolveMG.m:31-37 (0 V, 1 W, 2 FMV). */
 int or_solve_mg_cycle(const or_ctx *c, double *x, const double *f, int cycle, int nu1, int nu2,
                       double tol, int maxiter, double *relres);
 

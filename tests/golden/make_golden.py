@@ -4,10 +4,12 @@ where the reference tree is mounted; the GPU box never reads the reference).
 
 1. setup_fixtures.json -- problem-setup known answers from the reference's
    own problem library (Implementation/Tests/*.mat, MATLAB v5 files read with
-   scipy.io.loadmat, which executes nothing): the support and load rows, and
-   the resulting free-DOF list (as count + SHA-256 of its 1-based int64 bytes)
-   and the non-zero entries of the load vector str.f. Only files whose loads
-   are point or line loads are kept (the generator does not restate area loads).
+   scipy.io.loadmat, which executes nothing): for every file that keeps its
+   support and load rows, the rows, the fixed-density rows (str.fix), and the
+   resulting free-DOF list (count + SHA-256 of its 1-based int64 bytes), the
+   non-zero entries of the load vector str.f and the fixed elements
+   (str.fixedDens, count + SHA-256, solid count). stored_problems.json: the
+   same record (without rows) for the files saved without rows.
 2. small_cantilever.npz -- oracle outputs on an 8x4x4 cantilever (filtered
    random densities, seed 11, 3 levels): u, PCG count, compliance and
    gradient, each cross-checked here against an independent scipy solve.
@@ -37,31 +39,41 @@ def fd_hash(fd1):
 
 
 def setup_fixtures():
-    import scipy.io as sio
-    out = []
+    out, stored = [], []
     for path in sorted(glob.glob(os.path.join(REF_TESTS, "*.mat"))):
-        d = sio.loadmat(path, squeeze_me=True, struct_as_record=False)
+        d = sio_load(path)
         s = d["str"]
-        if "supp" not in s._fieldnames or "loads" not in s._fieldnames:
-            continue
-        loads = np.atleast_2d(np.asarray(s.loads, dtype=float))
-        if any(sum(r[2 * k + 1] > r[2 * k] for k in range(3)) > 1 for r in loads):
-            continue  # area load
-        supp = np.atleast_2d(np.asarray(s.supp, dtype=float))
+        names = set(s._fieldnames)
         fd = np.asarray(s.freedofs, dtype=np.int64).ravel()
         f = np.asarray(s.f, dtype=float).ravel()
         nz = np.nonzero(f)[0]
-        out.append({
-            "file": os.path.basename(path),
-            "nelx": int(s.nelx), "nely": int(s.nely), "nelz": int(s.nelz),
-            "l": float(s.l), "h": float(s.h), "w": float(s.w),
-            "supp": supp.tolist(), "loads": loads.tolist(),
-            "nfree": int(fd.size), "freedofs_sha256": fd_hash(fd),
-            "f_nonzero_index1": (nz + 1).tolist(), "f_nonzero_value": f[nz].tolist(),
-        })
+        fdens = np.asarray(getattr(s, "fixedDens", []), dtype=np.int64).ravel()
+        fval = np.asarray(getattr(s, "fixedDensVal", []), dtype=float).ravel()
+        rec = {"file": os.path.basename(path), "nelx": int(s.nelx), "nely": int(s.nely), "nelz": int(s.nelz),
+               "l": float(s.l), "h": float(s.h), "w": float(s.w),
+               "nfree": int(fd.size), "freedofs_sha256": fd_hash(fd),
+               "fixed_dens_count": int(fdens.size), "fixed_dens_sha256": fd_hash(fdens),
+               "fixed_dens_solid": int(np.sum(fval == 1))}
+        if "fix" in names and np.asarray(s.fix).size:
+            rec["fix"] = np.atleast_2d(np.asarray(s.fix, dtype=float)).tolist()
+        if "supp" in names and "loads" in names:
+            rec.update(supp=np.atleast_2d(np.asarray(s.supp, dtype=float)).tolist(),
+                       loads=np.atleast_2d(np.asarray(s.loads, dtype=float)).tolist(),
+                       f_nonzero_index1=(nz + 1).tolist(), f_nonzero_value=f[nz].tolist())
+            out.append(rec)
+        else:
+            rec.update(f_nonzero_count=int(nz.size), f_sum=float(f.sum()))
+            stored.append(rec)
     with open(os.path.join(HERE, "setup_fixtures.json"), "w") as fh:
         json.dump(out, fh, indent=1)
-    print(f"setup_fixtures.json: {len(out)} problems")
+    with open(os.path.join(HERE, "stored_problems.json"), "w") as fh:
+        json.dump(stored, fh, indent=1)
+    print(f"setup_fixtures.json: {len(out)} problems with rows; stored_problems.json: {len(stored)} without rows")
+
+
+def sio_load(path):
+    import scipy.io as sio
+    return sio.loadmat(path, squeeze_me=True, struct_as_record=False)
 
 
 def small_cantilever():
@@ -117,11 +129,64 @@ def large(grid, seed, levels):
     print(name, rec)
 
 
+def void_heavy(iters=30):
+    """Void-heavy designs from the oracle's own SLP loop (volfrac 0.3, linear
+    filter rmin 1.5): 48x24x24 single resolution (4 levels) and 24x12x12 with
+    2x2x2 density sub-elements (3 levels). For each: the filtered design, the
+    fraction of exactly-void density elements, the oracle's FP64 V-cycle and
+    W-cycle iteration counts and compliance at tol 1e-8, and per-level
+    operator probes (||A_l v||, v'A_l v for seeded v) for the GPU test."""
+    import time
+
+    import oracle
+    import slp_oracle
+    from tmg import problem as P
+    out = {}
+    arrays = {}
+    for name, grid, mr, L in (("sr48", (48, 24, 24), 1, 4), ("mr24", (24, 12, 12), 2, 3)):
+        prob = P.cantilever(*grid)
+        t0 = time.time()
+        hist, x, xfil, u = slp_oracle.str_slp(prob, 0.3, ngrids=L, mr=mr, maxiter=iters)
+        t_slp = time.time() - t0
+        o = oracle.Oracle(prob)
+        if mr == 1:
+            o.assemble(xfil)
+        else:
+            o.assemble_mr(mr, xfil)
+        o.setup_gmg(L, 0.6)
+        rec = {"grid": list(grid), "mr": mr, "levels": L, "slp_iterations": iters, "slp_s": t_slp,
+               "void_fraction": float(np.mean(xfil == 0.0)), "min_positive": float(xfil[xfil > 0].min()),
+               "slp_pcg_iterations": [int(h["pcgit"]) for h in hist]}
+        for cyc, cname in ((0, "V"), (1, "W")):
+            t0 = time.time()
+            uu, it, rr = o.solve(prob.f, cycle=cyc, maxiter=5000)
+            r = prob.f - o.spmv(0, uu)
+            rec[cname] = {"it": int(it), "relres": float(rr),
+                          "true_relres": float(np.linalg.norm(r) / np.linalg.norm(prob.f)),
+                          "compliance": float(prob.f @ uu), "solve_s": time.time() - t0}
+        probes = []
+        for lev in range(L):
+            v = np.random.default_rng(100 + lev).standard_normal(o.nfree(lev))
+            Av = o.spmv(lev, v)
+            probes.append({"norm_Av": float(np.linalg.norm(Av)), "vAv": float(v @ Av)})
+        rec["level_probes"] = probes
+        out[name] = rec
+        arrays[name + "_xfil"] = xfil
+        print(name, {k: v for k, v in rec.items() if k != "slp_pcg_iterations"})
+    with open(os.path.join(HERE, "void_heavy.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    np.savez_compressed(os.path.join(HERE, "void_heavy.npz"), **arrays)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true", help="also the 96x48x48 and 192x96x96 oracle solves")
     ap.add_argument("--only", default=None, help="only this large grid (e.g. 192x96x96)")
+    ap.add_argument("--void", action="store_true", help="only the void-heavy SLP designs (void_heavy.json/.npz)")
     a = ap.parse_args()
+    if a.void:
+        void_heavy()
+        sys.exit(0)
     if a.only:
         g = tuple(int(v) for v in a.only.split("x"))
         large(g, {96: 2, 192: 3}[g[0]], {96: 4, 192: 5}[g[0]])

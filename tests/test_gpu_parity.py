@@ -464,3 +464,25 @@ def test_symmetric_coarse_solve_matches_full_and_oracle(oracle_mod, monkeypatch,
     if precision == "fp64":
         assert abs(it - it_or) <= 1
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed", [(16, 8, 8, 3, 23), (16, 8, 8, 4, 24)])
+def test_fmv_cycle_matches_oracle(oracle_mod, precision, nelx, nely, nelz, L, seed):
+    """mg.cycle = 2: the FMV cycle of Implementation/FMVCycle.m as the
+    preconditioner (SolveMG.m:31-37), against the oracle's restatement."""
+    _say(f"fmv {precision} {nelx}x{nely}x{nelz} L={L}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    s = _setup(prob, x, L, precision, cycle=2)
+    b = np.random.default_rng(seed).standard_normal(prob.nfree)
+    assert _rel(s.vcycle(b), o.fmvcycle(b)) < (1e-8 if precision == "fp64" else 5e-5)
+    u_or, it_or, _ = o.solve(prob.f, cycle=2)
+    u, it, rr, rc = s.solve(prob.f)
+    assert rc == 0 and rr <= 1e-8
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    else:
+        assert it <= it_or + 3
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)

@@ -114,15 +114,69 @@ def _fixtures():
     return json.load(open(os.path.join(HERE, "golden", "setup_fixtures.json")))
 
 
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i8").tobytes()).hexdigest()
+
+
 @pytest.mark.parametrize("fx", _fixtures(), ids=lambda f: f["file"])
 def test_problem_generator_matches_reference_files(fx):
-    prob = P.from_rows(fx["nelx"], fx["nely"], fx["nelz"], fx["supp"], fx["loads"], l=fx["l"], h=fx["h"], w=fx["w"])
+    """Supports, point / line / area loads and fixed-density rows of the
+    reference's own problem files (24 of them keep their rows) rebuild their
+    stored str.freedofs, str.f and str.fixedDens / fixedDensVal."""
+    prob = P.from_rows(fx["nelx"], fx["nely"], fx["nelz"], fx["supp"], fx["loads"], l=fx["l"], h=fx["h"], w=fx["w"],
+                       fix=fx.get("fix"))
     assert prob.nfree == fx["nfree"]
-    h = hashlib.sha256(np.ascontiguousarray(prob.freedofs, dtype="<i8").tobytes()).hexdigest()
-    assert h == fx["freedofs_sha256"]
+    assert _sha(prob.freedofs) == fx["freedofs_sha256"]
     nz = np.nonzero(prob.f)[0]
     np.testing.assert_array_equal(nz + 1, fx["f_nonzero_index1"])
     np.testing.assert_allclose(prob.f[nz], fx["f_nonzero_value"], rtol=1e-15, atol=0)
+    assert prob.fixed_dens.size == fx["fixed_dens_count"] and _sha(prob.fixed_dens) == fx["fixed_dens_sha256"]
+    assert int(np.sum(prob.fixed_dens_val == 1)) == fx["fixed_dens_solid"]
+
+
+def test_fixture_library_covers_area_loads_and_fixed_regions():
+    fxs = _fixtures()
+    area = [f["file"] for f in fxs if any(sum(r[2 * k + 1] > r[2 * k] for k in range(3)) == 2 for r in f["loads"])]
+    fixed = [f["file"] for f in fxs if f.get("fix")]
+    assert len(fxs) == 24 and len(area) >= 2 and len(fixed) >= 4
+
+
+def test_area_load_shares():
+    """A load on an area spreads the line rule's shares in both directions
+    (Implementation/EnterData.m:350-782): a unit pressure on the face y = 0 of
+    a 4x2x3 block gives each node the product of its x and z shares (1/2 at
+    the ends, 1 inside) and totals the face area; loads on the faces z = 0
+    (xy plane) and x = 4 (yz plane) total theirs; a patch inside one node's
+    tributary area puts its whole area on that node."""
+    supp = [[0, 4, 2, 2, 0, 3, 1, 1, 1]]            # top face y = 2: no loaded node is supported
+    g = P.from_rows(4, 2, 3, supp, [[0, 4, 0, 0, 0, 3, 0, -1, 0]])
+    f = g.full(g.f).reshape(-1, 3)
+    nx, ny = 5, 3
+    wx, wz = [0.5, 1, 1, 1, 0.5], [0.5, 1, 1, 0.5]
+    for kz in range(4):
+        for kx in range(5):
+            assert f[kz * nx * ny + kx * ny, 1] == -wx[kx] * wz[kz]
+    assert abs(g.f.sum() + 12.0) < 1e-12 and np.count_nonzero(g.f) == 20
+    xy = P.from_rows(4, 2, 3, [[0, 4, 0, 2, 3, 3, 1, 1, 1]], [[0, 4, 0, 2, 0, 0, 0, 0, 2.0]])
+    assert abs(xy.f.sum() - 2.0 * 8) < 1e-12 and np.count_nonzero(xy.f) == 15
+    yz = P.from_rows(4, 2, 3, supp, [[4, 4, 0, 2, 0, 3, 1.0, 0, 0]])
+    assert abs(yz.f.sum() - (2 * 3 - 0.5 * 3)) < 1e-12          # the y = 2 edge is supported
+    h = P.from_rows(4, 2, 3, supp, [[1.1, 1.3, 0, 0, 2.2, 2.4, 0, -1, 0]])
+    nzv = h.f[np.nonzero(h.f)]
+    assert nzv.size == 1 and nzv[0] == -((1.3 - 1.1) * (2.4 - 2.2))
+    with pytest.raises(ValueError):
+        P.from_rows(4, 2, 3, supp, [[0, 4, 0, 2, 0, 3, 0, -1, 0]])   # volumetric
+
+
+def test_fixed_regions_element_layers():
+    """Void / solid regions by coordinates become element layers as
+    Implementation/EnterData.m:838-843 rounds them, clamped to the grid."""
+    rows, dens, val = P.fixed_regions(8, 4, 4, 1.0, 1.0, 1.0, [[0, 2, 0, 4, 0, 4, 0], [7.6, 9, 1.4, 2.6, 0, 1, 1]])
+    assert rows.tolist() == [[1, 2, 1, 4, 1, 4, 0], [8, 8, 1, 3, 1, 1, 1]]
+    assert dens.size == 2 * 4 * 4 + 3 and int(val.sum()) == 3
+    p = P.from_rows(8, 4, 4, [[0, 0, 0, 4, 0, 4, 1, 1, 1]], [[8, 8, 0, 0, 0, 4, 0, -1, 0]], fix=rows)
+    assert np.array_equal(p.fixed_dens, dens) and np.array_equal(p.fixed_dens_val, val)
+    assert p.free_dens_mask().sum() == p.nelem - dens.size
 
 
 def test_line_load_shares():
@@ -335,6 +389,49 @@ def test_wcycle_closed_forms(oracle_mod):
     assert b @ w > 0
 
 
+def test_fmv_cycle_closed_forms(oracle_mod):
+    """FMV cycle (Implementation/FMVCycle.m) pinned by its definition in
+    closed form: the V-cycle from an initial guess x0 is the affine step
+    x0 + B (b - A x0) of the zero-guess cycle; with two levels FMV is that
+    step from x0 = P A_c^-1 P' b (exact coarse solve); with three levels the
+    guess is P times the level-1 FMV of P' b, itself the level-1 step from
+    P1 A_2^-1 P1' (P' b). The FMV solve reaches the direct solution."""
+    prob = P.cantilever(8, 4, 4)
+    x = P.random_density(8, 4, 4, 8)
+    rng = np.random.default_rng(9)
+    for L in (2, 3):
+        o = oracle_mod.Oracle(prob)
+        o.assemble(x)
+        o.setup_gmg(L, 0.6)
+        n0 = o.nfree(0)
+        b = rng.standard_normal(n0)
+        x0 = rng.standard_normal(n0)
+        assert _rel(o.vcycle_x0(b, x0), x0 + o.vcycle(b - o.spmv(0, x0))) < 1e-11
+        assert np.array_equal(o.vcycle_x0(b, None), o.vcycle(b))
+        P0 = o.csr("P", 0).toarray()
+        if L == 2:
+            Ac = o.csr("A", 1).toarray()
+            guess = P0 @ np.linalg.solve(Ac, P0.T @ b)
+        else:
+            P1 = o.csr("P", 1).toarray()
+            A2 = o.csr("A", 2).toarray()
+            b1 = P0.T @ b
+            g1 = P1 @ np.linalg.solve(A2, P1.T @ b1)
+            x1 = g1 + o.vcycle(b1 - o.spmv(1, g1), level=1)
+            guess = P0 @ x1
+        ref = guess + o.vcycle(b - o.spmv(0, guess))
+        assert _rel(o.fmvcycle(b), ref) < 1e-9
+        assert _rel(o.fmvcycle(b), o.vcycle(b)) > 1e-6
+    prob = P.cantilever(16, 8, 8)
+    x = P.random_density(16, 8, 8, 7)
+    o = oracle_mod.Oracle(prob)
+    o.assemble(x)
+    o.setup_gmg(3, 0.6)
+    u_d = sla.spsolve(o.csr("A", 0).tocsc(), prob.f)
+    u_f, it_f, rr = o.solve(prob.f, cycle=2)
+    assert rr <= 1e-8 and _rel(u_f, u_d) < 1e-6
+
+
 def test_wcycle_solve_matches_direct(oracle_mod):
     prob = P.cantilever(16, 8, 8)
     x = P.random_density(16, 8, 8, 7)
@@ -493,8 +590,13 @@ def test_problem_file_round_trip(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir("/root/reference/Implementation/Tests"), reason="reference problem files absent")
 def test_reference_problem_files_read_back():
-    """Every reference problem file with point/line loads rebuilds to its
-    stored freedofs and f (the setup fixtures' source)."""
-    files = [fx["file"] for fx in json.load(open(os.path.join(HERE, "golden", "setup_fixtures.json")))]
-    for name in files[:8]:
-        P.from_mat(os.path.join("/root/reference/Implementation/Tests", name))
+    """All 73 reference problem files load: those with rows rebuild their
+    stored freedofs, f and fixed densities (checked inside from_mat), those
+    without give their stored vectors, equal to the committed records."""
+    stored = json.load(open(os.path.join(HERE, "golden", "stored_problems.json")))
+    rows = json.load(open(os.path.join(HERE, "golden", "setup_fixtures.json")))
+    assert len(stored) + len(rows) == 73
+    for fx in stored + rows:
+        p = P.from_mat(os.path.join("/root/reference/Implementation/Tests", fx["file"]))
+        assert p.nfree == fx["nfree"] and _sha(p.freedofs) == fx["freedofs_sha256"]
+        assert _sha(p.fixed_dens) == fx["fixed_dens_sha256"]

@@ -594,7 +594,38 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
     return (T *)l0.x;
 }
 
-bool fused_ok(const tmg_solver *h) { return h->fused && h->nsub == 1 && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1; }
+bool fused_ok(const tmg_solver *h) {
+    return h->fused && h->nsub == 1 && h->mg.cycle != 2 && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1;
+}
+
+// One FMV cycle at level l (Implementation/FMVCycle.m:21-39): restrict b,
+// solve the coarsest system directly or recurse, prolong, then one V-cycle
+// at level l from that initial guess. Returns the buffer holding the result.
+template <typename T>
+T *fmv(tmg_solver *h, int l, const T *b, const int *done) {
+    DevLevel &lv = h->lev[l];
+    DevLevel &lc = h->lev[l + 1];
+    hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz,
+                       lc.nx, lc.ny, lc.nz, b, lc.mask, (T *)lc.b, done);            // rH = P' b
+    T *xc;
+    if (l + 1 == h->L - 1) {
+        coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);                          // x = R \ (R' \ rH)
+        xc = (T *)lc.x;
+    } else {
+        xc = fmv<T>(h, l + 1, (const T *)lc.b, done);                                   // x = FMVCycle(..., nv + 1)
+    }
+    T *xa = (T *)lv.x;
+    hipLaunchKernelGGL((k_fill<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, T(0), xa);
+    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(lv.nx, lv.ny, lv.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny,
+                       lv.nz, lc.nx, lc.ny, lc.nz, (const T *)xc, lv.mask, xa, 1, done);   // x = P x
+    return vcycle<T>(h, l, b, done, xa);                                                // VCycle(A, b, x, ...)
+}
+
+// The preconditioner cycle of mg.cycle at the finest level: V, W or FMV.
+template <typename T>
+T *fine_cycle(tmg_solver *h, const T *b, const int *done) {
+    return h->mg.cycle == 2 ? fmv<T>(h, 0, b, done) : vcycle<T>(h, 0, b, done);
+}
 
 // Fine-level V(1,1) cycle on the FP64 PCG residual r with the conversion and
 // the PCG's r's folded into kernels the cycle runs anyway
@@ -631,7 +662,7 @@ T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done, bool x0_ready 
 
 // the fine level is matrix-free and the cycle is V(1,1) or W(1,1)
 bool fine_r_ok(const tmg_solver *h) {
-    return h->fine_r && !h->fine_stencil && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1;
+    return h->fine_r && !h->fine_stencil && h->mg.cycle != 2 && h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1;
 }
 
 // Preconditioner application s = B r for the PCG; r is the FP64 residual.
@@ -654,9 +685,9 @@ void precondition(tmg_solver *h, const double *r, const double **s64, const floa
         else *s64 = vcycle_fine_fused<double>(h, r, done);
     } else if (h->pprec == TMG_MIXED) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(l0.ndof)), dim3(TPB), 0, h->st, l0.ndof, r, (float *)l0.b, done);
-        *s32 = vcycle<float>(h, 0, (const float *)l0.b, done);
+        *s32 = fine_cycle<float>(h, (const float *)l0.b, done);
     } else {
-        *s64 = vcycle<double>(h, 0, r, done);
+        *s64 = fine_cycle<double>(h, r, done);
     }
 }
 
@@ -1218,8 +1249,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         g_setup_err = "invalid problem description";
         return TMG_ERR_ARG;
     }
-    if ((mg->cycle != 0 && mg->cycle != 1) || mg->smoother != 0 || mg->opchol != 0) {
-        g_setup_err = "only cycle=0 (V) or 1 (W), smoother=0 (Jacobi), opchol=0 (coarse Cholesky) are supported";
+    if (mg->cycle < 0 || mg->cycle > 2 || mg->smoother != 0 || mg->opchol != 0) {
+        g_setup_err = "only cycle=0 (V), 1 (W) or 2 (FMV), smoother=0 (Jacobi), opchol=0 (coarse Cholesky) are supported";
         return TMG_ERR_UNSUPPORTED;
     }
     if (mg->ngrids < 2 || mg->ngrids > MAXL || mg->smoothiter1 < 0 || mg->smoothiter2 < 0 || !(mg->omega > 0) ||

@@ -257,15 +257,30 @@ class _Prolongation:
         return self.s.restrict(self.level, v) if self.t else self.s.prolong(self.level, v)
 
 
+def _mr_factor(mr):
+    """mr.n of a multiresolution struct (1 without multiresolution); only
+    mr.n = mr.d = 2 with mr.interp = false is supported."""
+    if mr is None or not _get(mr, "op", False):
+        return 1
+    n, d = int(_get(mr, "n", 2)), int(_get(mr, "d", _get(mr, "n", 2)))
+    if n != 2 or d != 2 or _get(mr, "interp", False):
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "multiresolution is supported with mr.n = mr.d = 2, mr.interp = false")
+    return 2
+
+
 def GlobalStiffMatrix(str, kv, iK, jK, x, p, elem, emin, mr=None, strDens=None):
     """K = sum_e (emin + (E - emin) x_e^p) Ke restricted to freedofs
-    (Implementation/GlobalStiffMatrix.m:16-18,64-65). Returns a description of
-    K(x); the matrix is applied element by element on the device."""
-    if mr is not None and _get(mr, "op", False):
-        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "multiresolution stiffness is not supported")
+    (Implementation/GlobalStiffMatrix.m:16-18,64-65); with mr.op, each element
+    sums its 8 density sub-elements' integrands, x on the density mesh
+    (Implementation/GlobalStiffMatrix.m:19-62). Returns a description of K(x);
+    the matrix is applied element by element on the device."""
     if _get(elem, "deg", 1) != 1:
         raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only Q1 elements (elem.deg = 1) are supported")
-    return SimpleNamespace(str=str, x=np.asarray(x, dtype=np.float64), p=float(p), emin=float(emin))
+    n = _mr_factor(mr)
+    x = np.asarray(x, dtype=np.float64)
+    if x.size != str.nelem * n ** 3:
+        raise TmgError(_lib.TMG_ERR_ARG, "x does not match the element (or density) mesh")
+    return SimpleNamespace(str=str, x=x, p=float(p), emin=float(emin), mr=n)
 
 
 def SetupGMG(K, str, mg, elem, precision=None):
@@ -276,7 +291,7 @@ def SetupGMG(K, str, mg, elem, precision=None):
     s = Solver(str, int(_get(mg, "ngrids")), precision=prec, omega=float(_get(mg, "omega", 0.6)),
                smoothiter1=int(_get(mg, "smoothiter1", 1)), smoothiter2=int(_get(mg, "smoothiter2", 1)),
                tol=float(_get(mg, "tolMG", 1e-8)), maxiter=int(_get(mg, "maxiterMG", 5000)),
-               p=K.p, emin=K.emin, cycle=int(_get(mg, "cycle", 0)))
+               p=K.p, emin=K.emin, cycle=int(_get(mg, "cycle", 0)), mr=getattr(K, "mr", 1))
     if int(_get(mg, "smoother", 0)) != 0 or int(_get(mg, "opchol", 0)) != 0:
         s.close()
         raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only the Jacobi smoother and coarse Cholesky are supported")
@@ -295,8 +310,8 @@ def SolveMG(x, f, Ac, P, M, L, U, R, perm, ngrids, cycle, smoother, smoothiter1,
             tolMG, maxiterMG, opchol, opcond, pcgp=None):
     """[x,it] = SolveMG(...) (Implementation/SolveMG.m:1-55). The hierarchy's
     smoothing counts, tolerance and iteration limit are those given here."""
-    if cycle not in (0, 1) or smoother != 0 or opchol != 0:
-        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only cycle=0/1, smoother=0, opchol=0 are supported")
+    if cycle not in (0, 1, 2) or smoother != 0 or opchol != 0:
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "only cycle=0/1/2, smoother=0, opchol=0 are supported")
     if cycle != Ac.cycle:
         raise TmgError(_lib.TMG_ERR_ARG, "cycle differs from the hierarchy's (SetupGMG's mg.cycle)")
     if ngrids != Ac.ngrids:
@@ -322,6 +337,15 @@ def WCycle(A, b, x0, niter1, niter2, ngrids, nv, smoother, P, M, L, U, R, perm, 
         raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "WCycle is exposed at nv = 1 with x0 = 0, as SolveMG calls it")
     if A.cycle != 1:
         raise TmgError(_lib.TMG_ERR_ARG, "the hierarchy was set up for V-cycles (use VCycle)")
+    return A.vcycle(b)
+
+
+def FMVCycle(A, b, niter1, niter2, ngrids, nv, smoother, P, M, L, U, R, perm, opchol, opcond, pcgp=None):
+    """x = FMVCycle(A,b,niter1,niter2,ngrids,nv,...) (Implementation/FMVCycle.m:1-41) at the finest level."""
+    if nv != 1:
+        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "FMVCycle is exposed at nv = 1, as SolveMG calls it")
+    if A.cycle != 2:
+        raise TmgError(_lib.TMG_ERR_ARG, "the hierarchy was not set up for FMV cycles (mg.cycle = 2)")
     return A.vcycle(b)
 
 
@@ -351,10 +375,12 @@ def ApplyFilter(x, W, w, opfilter, trans):
 
 def Grad(str, u, p, emin, k, x, Dofs, W, w, opfilter, mr=None, strDens=None, A=None, freeDensG=None):
     """Fgrad = Grad(str,u,p,emin,k,x,Dofs,W,w,opfilter,mr,strDens)
-    (Implementation/Grad.m:1-64), single resolution. ``A`` is the hierarchy
-    whose handle runs the kernel (its p/emin must match)."""
-    if mr is not None and _get(mr, "op", False):
-        raise TmgError(_lib.TMG_ERR_UNSUPPORTED, "multiresolution gradient is not supported")
+    (Implementation/Grad.m:1-64); with mr.op the gradient lives on the
+    density mesh (Grad.m:24-49, mr.interp = false) and W is the density
+    mesh's filter. ``A`` is the hierarchy whose handle runs the kernel (its
+    p/emin and multiresolution factor must match)."""
+    if _mr_factor(mr) != getattr(A, "mr", 1):
+        raise TmgError(_lib.TMG_ERR_ARG, "mr does not match the hierarchy")
     if A is None:
         raise TmgError(_lib.TMG_ERR_ARG, "Grad needs the hierarchy handle A")
     freedens = None

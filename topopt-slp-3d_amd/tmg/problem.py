@@ -34,6 +34,22 @@ class Problem:
     f: np.ndarray = field(default=None)         # load on free DOFs (str.f)
     supp: np.ndarray = field(default=None)
     loads: np.ndarray = field(default=None)
+    fix: np.ndarray = field(default=None)       # fixed-density rows [ex0 exf ey0 eyf ez0 ezf val] (str.fix)
+    fixed_dens: np.ndarray = field(default=None)      # 1-based elements with fixed densities (str.fixedDens)
+    fixed_dens_val: np.ndarray = field(default=None)  # their values, 0 void / 1 solid (str.fixedDensVal)
+
+    def __post_init__(self):
+        if self.fixed_dens is None:
+            self.fixed_dens = np.zeros(0, dtype=np.int64)
+        if self.fixed_dens_val is None:
+            self.fixed_dens_val = np.zeros(0, dtype=np.float64)
+
+    def free_dens_mask(self) -> np.ndarray:
+        """0/1 per element: 1 where the density is a design variable
+        (str.freeDens, used as str.freeDensG by Implementation/Grad.m)."""
+        m = np.ones(self.nelem, dtype=np.uint8)
+        m[self.fixed_dens - 1] = 0
+        return m
 
     @property
     def nx(self) -> int:
@@ -94,11 +110,65 @@ def _line_nodes(c0: float, cf: float, h: float, F):
     return n0, [(k, [w * F[0], w * F[1], w * F[2]]) for k, w in out]
 
 
-def from_rows(nelx, nely, nelz, supp, loads, l=None, h=None, w=None, E=1.0, nu=0.3) -> Problem:
+def _element_layer(coord: float, h: float, n: int, lower: bool) -> int:
+    """1-based element layer of a fixed-density region bound
+    (Implementation/EnterData.m:838-843): the nearest layer boundary, clamped
+    to [1, n]."""
+    v = int(math.floor(coord / h)) + (1 if math.fmod(coord, h) > h / 2 else 0)
+    return max(v, 1) if lower else min(v, n)
+
+
+def fixed_regions(nelx, nely, nelz, el, eh, ew, regions):
+    """Rows ``[ex0 exf ey0 eyf ez0 ezf val]`` of str.fix and the fixed
+    elements from void (val 0) and solid (val 1) regions given by coordinates
+    ``[x0 xf y0 yf z0 zf val]`` (Implementation/EnterData.m:800-903). Returns
+    (fix rows, str.fixedDens 1-based, str.fixedDensVal)."""
+    fixd = np.zeros(nelx * nely * nelz, dtype=np.int8)
+    val = np.zeros(nelx * nely * nelz)
+    rows = []
+    for r in np.atleast_2d(np.asarray(regions, dtype=np.float64)) if len(regions) else []:
+        x0, xf, y0, yf, z0, zf, v = r
+        ex0, exf = _element_layer(x0, el, nelx, True), _element_layer(xf, el, nelx, False)
+        ey0, eyf = _element_layer(y0, eh, nely, True), _element_layer(yf, eh, nely, False)
+        ez0, ezf = _element_layer(z0, ew, nelz, True), _element_layer(zf, ew, nelz, False)
+        rows.append([ex0, exf, ey0, eyf, ez0, ezf, v])
+        _mark_fixed(fixd, val, nelx, nely, [ex0, exf, ey0, eyf, ez0, ezf, v])
+    dens = np.nonzero(fixd)[0]
+    return np.asarray(rows, dtype=np.float64).reshape(-1, 7), dens.astype(np.int64) + 1, val[dens]
+
+
+def _mark_fixed(fixd, val, nelx, nely, row):
+    ex0, exf, ey0, eyf, ez0, ezf, v = (int(row[0]), int(row[1]), int(row[2]), int(row[3]), int(row[4]), int(row[5]),
+                                       float(row[6]))
+    for ez in range(ez0, ezf + 1):
+        for ex in range(ex0, exf + 1):
+            for ey in range(ey0, eyf + 1):
+                e = (ey - 1) + (ex - 1) * nely + (ez - 1) * nelx * nely
+                fixd[e] = 1
+                if v == 1:
+                    val[e] = 1.0
+
+
+def fixed_from_rows(nelx, nely, nelz, fix):
+    """str.fixedDens and str.fixedDensVal from stored str.fix rows (element
+    layers, Implementation/EnterData.m:829-903)."""
+    fixd = np.zeros(nelx * nely * nelz, dtype=np.int8)
+    val = np.zeros(nelx * nely * nelz)
+    for r in np.atleast_2d(np.asarray(fix, dtype=np.float64)):
+        _mark_fixed(fixd, val, nelx, nely, r)
+    dens = np.nonzero(fixd)[0]
+    return dens.astype(np.int64) + 1, val[dens]
+
+
+def from_rows(nelx, nely, nelz, supp, loads, l=None, h=None, w=None, E=1.0, nu=0.3, fix=None) -> Problem:
     """Build freedofs and f from support rows ``[x0 xf y0 yf z0 zf ix iy iz]``
     and load rows ``[x0 xf y0 yf z0 zf Fx Fy Fz]``
-    (Implementation/EnterData.m:79-156 supports, :159-348 loads, :790-797 assembly).
-    Point loads and line loads along x, y or z are supported.
+    (Implementation/EnterData.m:79-156 supports, :159-348 point and line
+    loads, :350-782 loads on areas parallel to a coordinate plane, :790-797
+    assembly), and the fixed densities from ``fix`` rows
+    ``[ex0 exf ey0 eyf ez0 ezf val]`` (str.fix, :829-903). A load on an area
+    spreads the per-length shares of the line rule in both directions: each
+    node gets the product of its two one-dimensional shares times F.
     """
     l = float(nelx if l is None else l)
     h = float(nely if h is None else h)
@@ -146,38 +216,77 @@ def from_rows(nelx, nely, nelz, supp, loads, l=None, h=None, w=None, E=1.0, nu=0
         elif dz and not (dx or dy):
             n0, sh = _line_nodes(z0, zf, ew, F)
             shares = [(base(_layer(x0, el), _layer(y0, eh), n0 + k), v) for k, v in sh]
+        elif dx and dy and dz:
+            raise ValueError("volumetric loads are rejected (Implementation/EnterData.m:784-787)")
         else:
-            raise NotImplementedError("area loads are not supported by this generator")
+            one = (1.0, 1.0, 1.0)
+            ax = _line_nodes(x0, xf, el, one) if dx else (_layer(x0, el), [(0, [1.0, 1.0, 1.0])])
+            ay = _line_nodes(y0, yf, eh, one) if dy else (_layer(y0, eh), [(0, [1.0, 1.0, 1.0])])
+            az = _line_nodes(z0, zf, ew, one) if dz else (_layer(z0, ew), [(0, [1.0, 1.0, 1.0])])
+            shares = []
+            for kz, wz in az[1]:
+                for kx, wx in ax[1]:
+                    for ky, wy in ay[1]:
+                        # the two spanning directions' shares, multiplied first
+                        wgt = [wa for wa, on in ((wx[0], dx), (wy[0], dy), (wz[0], dz)) if on]
+                        sc = wgt[0] * wgt[1]
+                        shares.append((base(ax[0] + kx, ay[0] + ky, az[0] + kz), [sc * F[0], sc * F[1], sc * F[2]]))
         for node, v in shares:
             for c in range(3):
                 f[3 * node - 3 + c] += v[c]
+    fixed_dens = fixed_val = None
+    fixrows = None
+    if fix is not None and np.asarray(fix).size:
+        fixrows = np.atleast_2d(np.asarray(fix, dtype=np.float64))
+        fixed_dens, fixed_val = fixed_from_rows(nelx, nely, nelz, fixrows)
     return Problem(nelx, nely, nelz, el, eh, ew, E, nu, freedofs, f[freedofs - 1].copy(),
-                   supp, loads)
+                   supp, loads, fixrows, fixed_dens, fixed_val)
 
 
 def from_mat(path, check=True) -> Problem:
     """Read a reference problem file (the ``str`` struct of a ``.mat`` as the
     reference stores them, e.g. ``Implementation/Tests/cb48x16x16.mat``) with
-    ``scipy.io.loadmat``, which executes nothing, and rebuild it through
-    :func:`from_rows` from its support and load rows. With ``check`` the
-    rebuilt ``freedofs`` and ``f`` must equal the stored ones."""
+    ``scipy.io.loadmat``, which executes nothing. When the file keeps its
+    support and load rows (str.supp, str.loads) the problem is rebuilt from
+    them through :func:`from_rows`, and the fixed densities from str.fix; with
+    ``check`` the rebuilt freedofs, f and fixedDens / fixedDensVal must equal
+    the stored ones. Files without rows (entered interactively and saved
+    without them) give their stored freedofs, f and fixed densities."""
     import scipy.io as sio
     d = sio.loadmat(path, squeeze_me=True, struct_as_record=False)
     if "str" not in d:
         raise ValueError(f"{path}: no 'str' struct")
     st = d["str"]
     names = set(st._fieldnames)
-    for k in ("nelx", "nely", "nelz", "supp", "loads"):
+    for k in ("nelx", "nely", "nelz"):
         if k not in names:
             raise ValueError(f"{path}: str has no field {k!r}")
     get = lambda k, dflt=None: getattr(st, k) if k in names else dflt
-    prob = from_rows(int(st.nelx), int(st.nely), int(st.nelz), get("supp"), get("loads"),
-                     l=get("l"), h=get("h"), w=get("w"), E=float(get("E", 1.0)), nu=float(get("nu", 0.3)))
-    if check and "freedofs" in names and "f" in names:
-        fd = np.asarray(st.freedofs, dtype=np.int64).ravel()
-        f = np.asarray(st.f, dtype=np.float64).ravel()
-        if not (np.array_equal(fd, prob.freedofs) and np.allclose(f, prob.f, rtol=0, atol=1e-12)):
-            raise ValueError(f"{path}: stored freedofs/f differ from the ones rebuilt from supp/loads")
+    nelx, nely, nelz = int(st.nelx), int(st.nely), int(st.nelz)
+    fix = get("fix")
+    fix = None if fix is None or np.asarray(fix).size == 0 else np.atleast_2d(np.asarray(fix, dtype=np.float64))
+    stored_fd = np.asarray(get("fixedDens", []), dtype=np.int64).ravel()
+    stored_fv = np.asarray(get("fixedDensVal", []), dtype=np.float64).ravel()
+    if "supp" in names and "loads" in names:
+        prob = from_rows(nelx, nely, nelz, get("supp"), get("loads"), l=get("l"), h=get("h"), w=get("w"),
+                         E=float(get("E", 1.0)), nu=float(get("nu", 0.3)), fix=fix)
+        if check and "freedofs" in names and "f" in names:
+            fd = np.asarray(st.freedofs, dtype=np.int64).ravel()
+            f = np.asarray(st.f, dtype=np.float64).ravel()
+            if not (np.array_equal(fd, prob.freedofs) and np.allclose(f, prob.f, rtol=0, atol=1e-12)):
+                raise ValueError(f"{path}: stored freedofs/f differ from the ones rebuilt from supp/loads")
+    else:
+        l, h, w = (float(get(k, n)) for k, n in (("l", nelx), ("h", nely), ("w", nelz)))
+        prob = Problem(nelx, nely, nelz, l / nelx, h / nely, w / nelz, float(get("E", 1.0)), float(get("nu", 0.3)),
+                       np.asarray(st.freedofs, dtype=np.int64).ravel(), np.asarray(st.f, dtype=np.float64).ravel(),
+                       fix=fix)
+        if fix is not None:
+            prob.fixed_dens, prob.fixed_dens_val = fixed_from_rows(nelx, nely, nelz, fix)
+        else:
+            prob.fixed_dens, prob.fixed_dens_val = stored_fd, stored_fv
+    if check and "fixedDens" in names:
+        if not (np.array_equal(prob.fixed_dens, stored_fd) and np.array_equal(prob.fixed_dens_val, stored_fv)):
+            raise ValueError(f"{path}: stored fixedDens/fixedDensVal differ from the ones rebuilt from str.fix")
     return prob
 
 

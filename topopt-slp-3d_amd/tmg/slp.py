@@ -119,8 +119,17 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
     V = L * H * Wd
     velem = np.full(nelem, V / nelem)
     Vmax = volfrac * V
-    fixed = np.zeros(nelem, dtype=bool) if fixed_design is None else np.asarray(fixed_design, dtype=bool)
+    fixed_val = np.zeros(nelem)
+    if fixed_design is None and n == 1 and getattr(prob, "fixed_dens", None) is not None and prob.fixed_dens.size:
+        # the problem's void / solid regions (str.fixedDens, str.fixedDensVal)
+        fixed = np.zeros(nelem, dtype=bool)
+        fixed[prob.fixed_dens - 1] = True
+        fixed_val[prob.fixed_dens - 1] = prob.fixed_dens_val
+    else:
+        fixed = np.zeros(nelem, dtype=bool) if fixed_design is None else np.asarray(fixed_design, dtype=bool)
     freeD = ~fixed
+    # str.freeDensG = str.freeDens: fixed elements carry no gradient (StrSLP.m:75, Grad.m:18-23)
+    freedens = freeD.astype(np.uint8) if (fixed.any() and n == 1) else None
 
     t0 = time.perf_counter()
     own_filter = filt is None
@@ -149,11 +158,18 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
 
     def gradient(u_full, xfil):
         t = time.perf_counter()
-        g = solver.grad(u_full, xfil, filt=filt if opfilter != 0 else None)
+        g = solver.grad(u_full, xfil, freedens=freedens, filt=filt if opfilter != 0 else None)
         tm["grad"] += time.perf_counter() - t
         return g
 
-    x = np.full(nelem, float(volfrac)) if x0 is None else np.array(x0, dtype=np.float64)
+    if x0 is None:
+        # Implementation/StartStrSLP.m:61-64
+        x = np.full(nelem, float(volfrac))
+        if fixed.any():
+            x[fixed] = fixed_val[fixed]
+            x[freeD] = (volfrac * V - fixed_val[fixed].sum() * (L / dnx) * (H / dny) * (Wd / dnz)) / freeD.sum()
+    else:
+        x = np.array(x0, dtype=np.float64)
     delta = slp.delta0
     theta1 = theta2 = thetaMax = 1.0
     countS = countG = countF = 0
