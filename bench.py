@@ -45,6 +45,8 @@ def parse():
                     help="solve: MG-PCG solves + serving stream (default); slp: the multiresolution SLP loop")
     ap.add_argument("--slp-iters", type=int, default=30)
     ap.add_argument("--slp-grid", default="240x120x120", help="displacement mesh of the SLP workload")
+    ap.add_argument("--slp-cycle", type=int, default=1, choices=[0, 1, 2],
+                    help="mg.cycle of the SLP workload (1 = W, the reference's default, Implementation/TopOpt3D.m:172)")
     ap.add_argument("--probe-kernel", type=int, default=None,
                     help="profiling aid: only launch this solver kernel (tmg_time_kernel id) --kernel-reps times "
                          "on the workload and print its average; no JSON bench line")
@@ -121,17 +123,18 @@ def cpu_baseline(prob, x, L, gpu_iters):
 def iteration_bytes(prob, L, prec):
     """Algorithmic HBM bytes of one PCG iteration (each array the kernels read
     or write counted once per pass; no halo or cache effects), for the
-    solver's schedule: FP64 outer recurrences and q = K d; the fine level and
-    level 1 matrix-free (moduli only); assembled symmetric stencils (126
-    coefficients per node) below; the dense FP64 coarse inverse."""
+    solver's schedule: FP64 recurrences and q = K d (x updated one iteration
+    late inside the direction update, the cycle's first Jacobi sweep inside
+    the residual update); the fine level and level 1 matrix-free (moduli and
+    free-DOF mask only); assembled symmetric stencils (126 coefficients per
+    node) below; the dense FP64 coarse inverse."""
     s = 4 if prec == "mixed" else 8
     n0, ne0 = prob.ndof, prob.nelem
-    b = 8 * (2 * n0 + ne0)            # q = K d (read d, E; write q)
-    b += 8 * 6 * n0                   # x, r update with ||r||^2 (read x, r, d, q; write x, r)
-    b += (s + 16) * n0                # d = s + beta d
-    b += (8 + 2 * s) * n0             # first fine sweep from zero (read r, invD; write x)
-    b += s * (3 * n0 + ne0) + 8 * n0  # fine residual (read x, E, r; write)
-    b += (s * 4 + 8) * n0 + s * ne0   # fine post-sweep (read x, invD, E, r; write)
+    b = 8 * (2 * n0 + ne0) + n0       # q = K d (read d, E, mask; write q)
+    b += (24 + 2 * s) * n0            # r -= alpha q, ||r||^2, x0 = invD r (read r, q, invD; write r, x0)
+    b += (s + 32) * n0                # x += alpha d_old, d = s + beta d_old (read s, d, x; write d, x)
+    b += s * (2 * n0 + ne0) + 9 * n0  # fine residual (read x, E, r, mask; write)
+    b += s * (3 * n0 + ne0) + 8 * n0  # fine post-sweep (read x, invD, E, r; write)
     ex, ey, ez = prob.nelx, prob.nely, prob.nelz
     nprev = n0
     for l in range(1, L):
@@ -193,7 +196,7 @@ def slp_workload(args):
     log(f"SLP: {nelx}x{nely}x{nelz} MR 2, {prob.nfree} free DOFs, {8 * prob.nelem} design variables")
     t0 = time.perf_counter()
     res = slp.StrSLP(prob, 0.3, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, ngrids=4, precision="mixed", mr=2,
-                     slp=slp.SLPParams(maxiter=args.slp_iters), log=log)
+                     slp=slp.SLPParams(maxiter=args.slp_iters), log=log, cycle=args.slp_cycle)
     wall = time.perf_counter() - t0
     nsolve = res.ns
     line = {
@@ -205,9 +208,10 @@ def slp_workload(args):
         "config": {
             "workload": f"cantilever {nelx}x{nely}x{nelz} Q1 displacement mesh, multiresolution 2x2x2 density/design "
                         f"sub-elements ({2 * nelx}x{2 * nely}x{2 * nelz}), volfrac 0.3, linear filter rmin 1.5, "
-                        f"4-level GMG V(1,1), tol 1e-8",
+                        f"4-level GMG {'VWF'[args.slp_cycle]}(1,1) cycle, tol 1e-8, maxiterMG 5000",
             "n_free_dofs": prob.nfree, "design_variables": 8 * prob.nelem,
-            "iterations": res.iter, "rejected": res.itrej, "solves": nsolve, "pcg_iterations": res.pcgit,
+            "mg_cycle": "VWF"[args.slp_cycle], "iterations": res.iter, "rejected": res.itrej, "solves": nsolve,
+            "pcg_iterations": res.pcgit, "unconverged_solves": int(sum(1 for it in res.pcgit if it > 5000)),
             "F": res.F, "vol": res.vol, "opstop": res.opstop, "wall_s": round(wall, 3),
             "time_s": {kk: round(v, 3) for kk, v in res.time.items()},
             "F_history": [float(f"{h['F']:.8g}") for h in res.history],

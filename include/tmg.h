@@ -207,8 +207,8 @@ int tmg_lp_volume(int64_t n, const double *c, const double *a, const double *sL,
  * forms, preconditioner precision). Requires a prior tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
-/* Instrumentation (no counterpart in the reference): device bytes the handle
- * holds, each allocation rounded up to 256 bytes. */
+/* Instrumentation (no counterpart in the reference): the most device bytes
+ * the handle has held at once, each allocation rounded up to 256 bytes. */
 int64_t tmg_device_bytes(const tmg_solver *h);
 
 /* ---------------------------------------------------------------------------

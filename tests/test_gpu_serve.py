@@ -102,3 +102,66 @@ def test_failed_request_is_reported():
     rid2 = srv.submit(r["prob"], r["x"], r["ngrids"])   # the server keeps serving
     assert srv.wait(rid2, 60)[0] and srv.result(rid2)["rc"] == 0
     srv.close()
+
+
+def test_served_results_match_oracle(oracle_mod):
+    """A mix of uneven requests including a 128x64x64 grid (1.6M DOFs,
+    5 levels): every served result meets the oracle's operator (true
+    residual), its compliance and its iteration count, and the large one is
+    bitwise the same as when served alone."""
+    from tmg import serve
+    rng = np.random.default_rng(51)
+    reqs = []
+    for i, n in enumerate((32, 64, 128, 48)):
+        prob = P.cantilever(n, n // 2, n // 2, load="point", tip=(int(rng.integers(0, n // 2 + 1)),
+                                                                   int(rng.integers(0, n // 2 + 1))))
+        L, _, ok = P.choose_ngrids(n, n // 2, n // 2, 5)
+        reqs.append({"prob": prob, "x": P.random_density(n, n // 2, n // 2, 2000 + i), "ngrids": L})
+    reqs.append(dict(reqs[3], precision="fp64"))      # the 48x24x24 request again, FP64 preconditioner
+    srv = serve.Server(budget_bytes=40e9, max_active=5)
+    ids = [srv.submit(r["prob"], r["x"], r["ngrids"], precision=r.get("precision", "mixed")) for r in reqs]
+    res = []
+    for rid in ids:
+        assert srv.wait(rid, 300)[0]
+        res.append(srv.result(rid))
+    srv.close()
+    for r, m in zip(reqs, res):
+        prob = r["prob"]
+        _say(f"served {prob.nelx}x{prob.nely}x{prob.nelz}: it {m['it']}")
+        o = oracle_mod.Oracle(prob)
+        o.assemble(r["x"])
+        o.setup_gmg(r["ngrids"], 0.6)
+        u_or, it_or, _ = o.solve(prob.f)
+        relres = np.linalg.norm(prob.f - o.spmv(0, m["u"])) / np.linalg.norm(prob.f)
+        C, Co = prob.f @ m["u"], prob.f @ u_or
+        _say(f"  oracle it {it_or}, true relres {relres:.2e}, compliance err {abs(C - Co) / abs(Co):.2e}")
+        assert m["rc"] == 0 and relres <= 1.2e-8 and abs(C - Co) / abs(Co) < 1e-7
+        if r.get("precision") == "fp64":
+            assert abs(m["it"] - it_or) <= 1
+        else:   # the FP32 cycle's rounding may cost iterations, never the stop rule
+            assert m["it"] <= 1.3 * it_or + 3
+    big = reqs[2]
+    alone = serve.Server(budget_bytes=40e9, max_active=1)
+    rid = alone.submit(big["prob"], big["x"], big["ngrids"])
+    assert alone.wait(rid, 300)[0]
+    a = alone.result(rid)
+    alone.close()
+    assert a["it"] == res[2]["it"] and np.array_equal(a["u"], res[2]["u"])
+
+
+def test_estimate_covers_allocated_bytes():
+    """The admission estimate is an upper bound of what a handle allocates
+    (tmg_device_bytes after set-up, a density update and a solve), including
+    a problem whose coarse system takes the symmetric coarse solve."""
+    import tmg
+    from tmg import serve
+    for n, L in ((32, 5), (64, 4), (56, 2)):
+        prob = P.cantilever(n, n // 2, n // 2, load="point", tip=(0, 0))
+        for prec in ("fp64", "mixed"):
+            s = tmg.Solver(prob, L, precision=prec)
+            s.update_density(P.random_density(n, n // 2, n // 2, 3))
+            s.solve(prob.f)
+            est = serve.estimate_bytes(prob, L, prec)
+            _say(f"estimate {n} L={L} {prec}: {est} >= {s.device_bytes()}")
+            assert est >= s.device_bytes()
+            s.close()

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace tmg;
@@ -155,7 +156,9 @@ struct tmg_solver {
     double *stage = nullptr;          // pinned host staging lent by a server slot: host vectors pass through it
     size_t stage_cap = 0;             // its capacity in doubles
     bool own_stream = true, own_rb = true, own_hsc = true;
-    size_t bytes = 0;                 // device bytes held by this handle
+    size_t bytes = 0;                 // device bytes held by this handle now
+    size_t peak_bytes = 0;            // and at most, over its lifetime
+    std::unordered_map<void *, size_t> sizes;   // rounded size of each live allocation
     bool have_op = false;
     std::string err;
 };
@@ -200,6 +203,8 @@ int dalloc(tmg_solver *h, T **p, int64_t n) {
     }
     if (poison_buffers() && h->st) (void)hipMemsetAsync((void *)*p, 0xFF, bytes, h->st);
     h->bytes += round256(bytes);
+    h->peak_bytes = std::max(h->peak_bytes, h->bytes);
+    h->sizes[(void *)*p] = round256(bytes);
     return 0;
 }
 
@@ -208,6 +213,11 @@ void dfree(tmg_solver *h, T *&p) {
     if (p) {
         if (h->pool) (void)hipFreeAsync((void *)p, h->st);
         else (void)hipFree((void *)p);
+        auto it = h->sizes.find((void *)p);
+        if (it != h->sizes.end()) {
+            h->bytes -= it->second;
+            h->sizes.erase(it);
+        }
     }
     p = nullptr;
 }
@@ -1868,7 +1878,7 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     return TMG_OK;
 }
 
-int64_t tmg_device_bytes(const tmg_solver *h) { return h ? (int64_t)h->bytes : 0; }
+int64_t tmg_device_bytes(const tmg_solver *h) { return h ? (int64_t)h->peak_bytes : 0; }
 
 int tmg_solve_stats(const tmg_solver *h, int *escalated_at, int *escalations) {
     if (!h) return TMG_ERR_ARG;

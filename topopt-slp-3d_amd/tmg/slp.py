@@ -104,12 +104,13 @@ class SLPResult:
 
 
 def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, slp=None, ngrids=4,
-           precision="mixed", mr=1, x0=None, fixed_design=None, log=print, solver=None, filt=None):
+           precision="mixed", mr=1, x0=None, fixed_design=None, log=print, solver=None, filt=None, cycle=0):
     """Minimum compliance by SLP (Implementation/StrSLP.m:1-605) with the GMG
     solve on the device. ``mr = 2`` is multiresolution with ``mr.n = mr.d = 2``
     (densities and design variables on the same 2x finer mesh). ``x0`` is the
     initial design (default ``volfrac`` everywhere, Implementation/StartStrSLP.m:61).
-    ``solver``/``filt`` may pass an existing hierarchy and filter to reuse."""
+    ``solver``/``filt`` may pass an existing hierarchy and filter to reuse;
+    ``cycle`` is mg.cycle (0 V, 1 W, the reference's default, 2 FMV)."""
     slp = slp or SLPParams()
     tm = dict(setupK=0.0, setupPrec=0.0, system=0.0, grad=0.0, LP=0.0, filter=0.0)
     n = int(mr)
@@ -138,7 +139,7 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
                       fixed=fixed.astype(np.uint8) if fixed.any() else None)
     own_solver = solver is None
     if own_solver:
-        solver = Solver(prob, ngrids, precision=precision, p=p, emin=emin, mr=n)
+        solver = Solver(prob, ngrids, precision=precision, p=p, emin=emin, mr=n, cycle=cycle)
     tm["setupPrec"] += time.perf_counter() - t0
 
     def apply_filter(v, trans=False):
