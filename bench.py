@@ -52,8 +52,8 @@ def parse():
                          "on the workload and print its average; no JSON bench line")
     ap.add_argument("--probe-precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="profiling aid: only run an elementwise kernel that reads and writes this many bytes "
-                         "(calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE)")
+                    help="profiling aid: only run three elementwise kernels (scale, copy, add) on vectors of this "
+                         "many bytes (calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -157,17 +157,26 @@ def probe(args):
     --kernel-reps times; prints its HIP-event average. Kernel ids are those of
     tmg_time_kernel (0 FP64 fine apply q = K d, 1 fine Jacobi sweep, 2 level-2
     sweep, 3 V-cycle, 4/5 fused fine residual / post-sweep, 6 fused direction
-    update + apply)."""
+    update + apply, 7/8 fine residual / post-sweep on the FP64 PCG residual)."""
     import torch
     torch.cuda.init()
     if args.copy_bytes:
+        # three kernels of known HBM traffic on FP32 vectors of copy_bytes
+        # each: scale (read 1, write 1), copy (read 1, write 1), add (read 2,
+        # write 1); their counter readings calibrate FETCH_SIZE / WRITE_SIZE
         n = args.copy_bytes // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
+        src2 = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(args.kernel_reps):
             torch.mul(src, 2.0, out=dst)
+        for _ in range(args.kernel_reps):
+            dst.copy_(src2)
+        for _ in range(args.kernel_reps):
+            torch.add(src, src2, out=dst)
         torch.cuda.synchronize()
-        log(json.dumps({"copy_bytes": args.copy_bytes, "reps": args.kernel_reps}))
+        log(json.dumps({"copy_bytes": args.copy_bytes, "reps": args.kernel_reps,
+                        "kernels": ["mul (r1 w1)", "copy (r1 w1)", "add (r2 w1)"]}))
         return
     import tmg
     from tmg import problem as P
@@ -311,6 +320,10 @@ def main():
     reps = args.kernel_reps
     k = {
         "fine_apply_fp64_ms": sf.time_kernel(0, reps),
+        "fine_resid_fp32_ms": sm.time_kernel(7, reps),
+        "fine_post_fp32_ms": sm.time_kernel(8, reps),
+        "fine_resid_fp64_ms": sf.time_kernel(7, reps),
+        "fine_post_fp64_ms": sf.time_kernel(8, reps),
         "fine_jacobi_fp32_ms": sm.time_kernel(1, reps),
         "fine_jacobi_fp64_ms": sf.time_kernel(1, reps),
         "l2_jacobi_fp32_ms": sm.time_kernel(2, reps) if L >= 3 else None,
@@ -318,6 +331,15 @@ def main():
         "vcycle_mixed_ms": sm.time_kernel(3, max(reps // 5, 3)),
         "vcycle_fp64_ms": sf.time_kernel(3, max(reps // 5, 3)),
     }
+    # the same two kernels timed inside a solve (HIP events around each of
+    # their launches in one eagerly launched mixed solve): the back-to-back
+    # loops above keep their ~100 MB working set in the 256 MB Infinity
+    # Cache, the solve does not, so these are the figures of the solve
+    t_apply_is, n_apply_is = sm.time_in_solve(0)
+    t_post_is, n_post_is = sm.time_in_solve(8)
+    k["fine_apply_fp64_in_solve_ms"] = t_apply_is
+    k["fine_post_fp32_in_solve_ms"] = t_post_is
+    k["in_solve_launches"] = n_apply_is
     log("kernels:", json.dumps(k))
     # Dominant solve kernel: the FP64 fine-level operator apply q = K(rho) d
     # with the PCG dot d'q folded in (once per iteration in both modes).
@@ -328,25 +350,31 @@ def main():
     # is 1152).
     flops_apply = 225 * n_el
     bytes_apply64 = 8 * 2 * n_dof + 8 * n_el
-    t = k["fine_apply_fp64_ms"] * 1e-3
+    t = k["fine_apply_fp64_in_solve_ms"] * 1e-3
     achieved = bytes_apply64 / t / 1e9
     roofline = {
-        "bound": "hbm", "kernel": "k_ebe_wht<double,DOT> (fine-level FP64 operator apply q = K d with d'q, spectral element form)",
+        "bound": "hbm", "kernel": "k_ebe_lds<double,DOT> (fine-level FP64 operator apply q = K d with d'q, spectral element form)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": profile_traffic("k_ebe_wht_apply_fp64_cantilever192x96x96_hbm_traffic.txt"),
-        "bytes_per_launch": bytes_apply64, "avg_launch_ms": round(k["fine_apply_fp64_ms"], 5),
+        "traffic": profile_traffic("k_ebe_lds_apply_fp64_cantilever192x96x96_hbm_traffic.txt"),
+        "bytes_per_launch": bytes_apply64, "avg_launch_ms": round(k["fine_apply_fp64_in_solve_ms"], 5),
+        "avg_launch_ms_measured": f"HIP events on the solver's stream around each of the {n_apply_is} launches "
+                                  f"in one mixed solve",
+        "avg_launch_ms_back_to_back": round(k["fine_apply_fp64_ms"], 5),
         "valu_tflops": round(flops_apply / t / 1e12, 2), "valu_peak_tflops": FP64_VEC_PEAK_TFLOPS,
         "valu_frac": round(flops_apply / t / 1e12 / FP64_VEC_PEAK_TFLOPS, 4),
     }
-    # the preconditioner's fine-level sweep (FP32): read x, b, invD, E; write x
-    bytes_jac32 = 4 * 4 * n_dof + 4 * n_el
-    t32 = k["fine_jacobi_fp32_ms"] * 1e-3
+    # the preconditioner's fine-level post-sweep (FP32), the largest fine
+    # V-cycle kernel: x += omega invD (r - K x) with r the FP64 PCG residual;
+    # read x, invD (4 B), r (8 B), E (4 B per element); write x
+    bytes_post32 = (4 + 4 + 8 + 4) * n_dof + 4 * n_el
+    t32 = k["fine_post_fp32_in_solve_ms"] * 1e-3
     secondary = {
-        "kernel": "k_ebe_wht<float,JACOBI> (fine-level fused apply + damped Jacobi sweep, FP32)",
-        "achieved_gbs": round(bytes_jac32 / t32 / 1e9, 1), "frac": round(bytes_jac32 / t32 / 1e9 / HBM_PEAK_GBS, 4),
-        "bytes_per_launch": bytes_jac32, "avg_launch_ms": round(k["fine_jacobi_fp32_ms"], 5),
-        "traffic": profile_traffic("k_ebe_wht_jacobi_fp32_cantilever192x96x96_hbm_traffic.txt"),
+        "kernel": "k_ebe_lds<float,POSTR> (fine-level fused apply + damped Jacobi post-sweep on the FP64 residual, FP32)",
+        "achieved_gbs": round(bytes_post32 / t32 / 1e9, 1), "frac": round(bytes_post32 / t32 / 1e9 / HBM_PEAK_GBS, 4),
+        "bytes_per_launch": bytes_post32, "avg_launch_ms": round(k["fine_post_fp32_in_solve_ms"], 5),
+        "avg_launch_ms_back_to_back": round(k["fine_post_fp32_ms"], 5),
+        "traffic": profile_traffic("k_ebe_lds_post_fp32_cantilever192x96x96_hbm_traffic.txt"),
         "valu_tflops": round(flops_apply / t32 / 1e12, 2), "valu_peak_tflops": FP32_VEC_PEAK_TFLOPS,
         "valu_frac": round(flops_apply / t32 / 1e12 / FP32_VEC_PEAK_TFLOPS, 4),
     }

@@ -207,6 +207,17 @@ int tmg_lp_volume(int64_t n, const double *c, const double *a, const double *sL,
  * forms, preconditioner precision). Requires a prior tmg_update_density. */
 int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms);
 
+/* Instrumentation (no counterpart in the reference): average device time in
+ * milliseconds of kernel which (0 = the FP64 fine-level apply q = K d with
+ * d'q, 8 = the fine-level post-sweep on the FP64 PCG residual) as it runs
+ * inside a solve: one solve from x0 = 0 on the right-hand side of the last
+ * solve, launched without the graph, with HIP events on the handle's stream
+ * around each launch of that kernel in the iterations the PCG ran (*nlaunch
+ * of them). Unlike tmg_time_kernel's back-to-back launches, the kernel's
+ * inputs here have passed through the caches the way the solve leaves them.
+ * Overwrites the handle's solution vector. */
+int tmg_time_in_solve(tmg_solver *h, int which, double *avg_ms, int *nlaunch);
+
 /* Instrumentation (no counterpart in the reference): the most device bytes
  * the handle has held at once, each allocation rounded up to 256 bytes. */
 int64_t tmg_device_bytes(const tmg_solver *h);

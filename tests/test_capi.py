@@ -30,7 +30,7 @@ def _setup_rc(prob, ngrids=3, cycle=0, smoother=0, opchol=0, nu1=1, nu2=1, omega
 
 
 @pytest.mark.parametrize("kw,code", [
-    (dict(cycle=2), _lib.TMG_ERR_UNSUPPORTED),        # FMV cycle
+    (dict(cycle=3), _lib.TMG_ERR_UNSUPPORTED),        # no fourth cycle type (0 V, 1 W, 2 FMV)
     (dict(smoother=2), _lib.TMG_ERR_UNSUPPORTED),     # SSOR
     (dict(opchol=1), _lib.TMG_ERR_UNSUPPORTED),       # PCG on the coarsest grid
     (dict(ngrids=1), _lib.TMG_ERR_ARG),

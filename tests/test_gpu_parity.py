@@ -486,3 +486,25 @@ def test_fmv_cycle_matches_oracle(oracle_mod, precision, nelx, nely, nelz, L, se
     else:
         assert it <= it_or + 3
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+
+
+def test_time_in_solve_counts_the_iterations_run():
+    """tmg_time_in_solve times each launch of the FP64 apply and of the fine
+    post-sweep in one eagerly launched solve: one of each per PCG iteration,
+    none from the iterations skipped past convergence; the timed solve gives
+    the same solution as the graph-replayed one."""
+    _say("time in solve")
+    import tmg
+    prob = P.cantilever(32, 16, 16)
+    x = P.random_density(32, 16, 16, 7)
+    for prec in ("fp64", "mixed"):
+        s = _setup(prob, x, 4, prec)
+        u, it, _, _ = s.solve(prob.f)
+        ms0, n0 = s.time_in_solve(0)
+        ms8, n8 = s.time_in_solve(8)
+        assert n0 == it - 1 and n8 == it - 1 and ms0 > 0 and ms8 > 0
+        with pytest.raises(tmg.TmgError):
+            s.time_in_solve(3)
+        u2, it2, _, _ = s.solve(prob.f)
+        assert it2 == it and np.array_equal(u, u2)
+        s.close()

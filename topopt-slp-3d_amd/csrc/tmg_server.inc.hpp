@@ -248,6 +248,7 @@ void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
             if (!rc) rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
             if (rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
             if (!rc && (rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) rc = upload_free(h, 0, r->f.data(), h->pf);
+            h->have_rhs = true;
             if (!rc) rc = solve_begin(h);
             if (rc) msg = h->err;
         }

@@ -160,6 +160,12 @@ struct tmg_solver {
     size_t peak_bytes = 0;            // and at most, over its lifetime
     std::unordered_map<void *, size_t> sizes;   // rounded size of each live allocation
     bool have_op = false;
+    bool have_rhs = false;            // pf holds a right-hand side (a solve was started)
+    // kernel timing inside a solve (tmg_time_in_solve): event pairs recorded
+    // around each launch of kernel kt_which while the PCG runs eagerly
+    int kt_which = -1;
+    int kt_n = 0;
+    std::vector<hipEvent_t> kt_ev;
     std::string err;
 };
 
@@ -181,6 +187,20 @@ namespace {
     } while (0)
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// kernels timed inside a solve by tmg_time_in_solve (ids of tmg_time_kernel)
+constexpr int KT_APPLY = 0;        // FP64 fine apply q = K d with d'q
+constexpr int KT_POST = 8;         // fine post-sweep on the FP64 residual
+constexpr int KT_MAX_PAIRS = 1024;
+
+// Records the start (end = false) or the end of one launch of kernel id on
+// the solver's stream while tmg_time_in_solve runs (never inside a graph
+// capture: the timed solve runs eagerly).
+inline void kt_mark(tmg_solver *h, int id, bool end) {
+    if (h->kt_which != id || h->kt_n >= (int)h->kt_ev.size() / 2) return;
+    (void)hipEventRecord(h->kt_ev[2 * h->kt_n + (end ? 1 : 0)], h->st);
+    if (end) ++h->kt_n;
+}
 
 inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -666,7 +686,9 @@ T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done, bool x0_ready 
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.x = xa; b.r = r; b.out = xb; b.partial = h->partial; b.done = done;
     if (h->fin_fused) { b.sc = h->d_sc; b.fin = FIN_RHO; }       // rho = r's, beta: in the last block
+    kt_mark(h, KT_POST, false);
     launch_ebe<T, T, SRC_X, EPI_POSTR>(h, b);
+    kt_mark(h, KT_POST, true);
     return xb;
 }
 
@@ -766,13 +788,16 @@ void pcg_iteration(tmg_solver *h, int parity) {
         } else {
             hipLaunchKernelGGL((k_dir_update<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->d_sc);
         }
+        kt_mark(h, KT_APPLY, false);
         if (h->fin_fused && !h->fine_stencil) {
             // q = A d, d'q, alpha in the apply's last block
             EbeArgs<double, double> a = ebe_args<double, double>(h);
             a.x = h->pd; a.out = h->pq; a.partial = h->partial; a.done = done; a.sc = h->d_sc; a.fin = FIN_ALPHA;
             launch_ebe<double, double, SRC_X, EPI_DOT>(h, a);
+            kt_mark(h, KT_APPLY, true);
         } else {
             apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);      // q = A d, d'q
+            kt_mark(h, KT_APPLY, true);
             hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr,
                                h->d_sc);
         }
@@ -865,7 +890,7 @@ int solve_begin(tmg_solver *h) {
 // Launch nchunk x GRAPH_ITERS iterations, then an asynchronous copy of the
 // scalars to pinned memory and the completion event.
 int solve_launch(tmg_solver *h, int nchunk) {
-    hipGraphExec_t ge = h->pprec == h->prec ? h->graph_exec : h->graph_exec64;
+    hipGraphExec_t ge = !h->use_graph ? nullptr : h->pprec == h->prec ? h->graph_exec : h->graph_exec64;
     if (ge) {
         for (int k = 0; k < nchunk; ++k) HIPX(hipGraphLaunch(ge, h->st));
     } else {
@@ -1031,6 +1056,8 @@ void destroy_all(tmg_solver *h) {
     h->h_sc = nullptr;
     if (h->ev) (void)hipEventDestroy(h->ev);
     h->ev = nullptr;
+    for (hipEvent_t e : h->kt_ev) (void)hipEventDestroy(e);
+    h->kt_ev.clear();
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     h->graph_exec = nullptr;
@@ -1653,6 +1680,7 @@ int tmg_solve_device(tmg_solver *h, double *d_x, const double *d_f, int *it, dou
     const int64_t n = h->lev[0].ndof;
     HIPX(hipMemcpyAsync(h->px, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
     HIPX(hipMemcpyAsync(h->pf, d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    h->have_rhs = true;
     int iters = 0;
     double rr = 0.0;
     int rc = solve_resident(h, &iters, &rr);
@@ -1668,6 +1696,7 @@ int tmg_solve(tmg_solver *h, const double *x0, const double *f, double *x_out, i
     if (!h || !x0 || !f || !x_out) return TMG_ERR_ARG;
     TRY(upload_free(h, 0, x0, h->px));
     TRY(upload_free(h, 0, f, h->pf));
+    h->have_rhs = true;
     int iters = 0;
     double rr = 0.0;
     int rc = solve_resident(h, &iters, &rr);
@@ -1875,6 +1904,43 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     (void)hipEventDestroy(e1);
     TRY(check_launch(h));
     *avg_ms = (double)ms / nrep;
+    return TMG_OK;
+}
+
+int tmg_time_in_solve(tmg_solver *h, int which, double *avg_ms, int *nlaunch) {
+    if (!h || !avg_ms || (which != KT_APPLY && which != KT_POST)) return TMG_ERR_ARG;
+    if (!h->have_op) return set_err(h, TMG_ERR_STATE, "no operator: call tmg_update_density first");
+    if (!h->have_rhs) return set_err(h, TMG_ERR_STATE, "no right-hand side: solve once first");
+    if (!xdefer_ok(h) || fused_ok(h))
+        return set_err(h, TMG_ERR_UNSUPPORTED, "in-solve timing covers the default fine-level schedule only");
+    if (h->kt_ev.empty()) {
+        h->kt_ev.resize(2 * KT_MAX_PAIRS, nullptr);
+        for (auto &e : h->kt_ev) HIPX(hipEventCreate(&e));
+    }
+    const int64_t n = h->lev[0].ndof;
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, 0.0, h->px);
+    const bool graph = h->use_graph;
+    h->use_graph = false;
+    h->kt_which = which;
+    h->kt_n = 0;
+    int it = 0;
+    double rr = 0.0;
+    const int rc = solve_resident(h, &it, &rr);
+    h->kt_which = -1;
+    h->use_graph = graph;
+    if (rc < 0) return rc;
+    HIPX(hipStreamSynchronize(h->st));
+    // the PCG ran it - 1 iterations (the count starts at 1, as the
+    // reference's); launches of the iterations past convergence exit at once
+    const int m = std::min(h->kt_n, std::max(it - 1, 0));
+    double sum = 0.0;
+    for (int i = 0; i < m; ++i) {
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, h->kt_ev[2 * i], h->kt_ev[2 * i + 1]));
+        sum += ms;
+    }
+    *avg_ms = m > 0 ? sum / m : 0.0;
+    if (nlaunch) *nlaunch = m;
     return TMG_OK;
 }
 

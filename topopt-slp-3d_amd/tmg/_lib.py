@@ -82,6 +82,7 @@ _SIGS = {
     "tmg_filter_rowsum": (_i, [_p, _pd]),
     "tmg_filter_destroy": (None, [_p]),
     "tmg_time_kernel": (_i, [_p, _i, _i, _pd]),
+    "tmg_time_in_solve": (_i, [_p, _i, _pd, _pi]),
     "tmg_device_bytes": (C.c_int64, [_p]),
     "tmg_lp_volume": (_i, [C.c_int64, _pd, _pd, _pd, _pd, _d, _i, _pd, _pd]),
     "tmg_server_create": (_i, [_d, _i, C.POINTER(_p)]),

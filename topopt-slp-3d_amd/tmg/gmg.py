@@ -187,6 +187,14 @@ class Solver:
         self._chk(lib().tmg_time_kernel(self.h, which, nrep, C.byref(ms)))
         return ms.value
 
+    def time_in_solve(self, which):
+        """Average launch time (ms) of a solver kernel (0 FP64 apply, 8 fine
+        post-sweep) inside one solve on the last right-hand side, and the
+        number of launches averaged; overwrites the handle's solution."""
+        ms, n = C.c_double(), C.c_int()
+        self._chk(lib().tmg_time_in_solve(self.h, which, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def grad(self, u_full, x, freedens=None, filt=None):
         u_full, x = _f64(u_full), _f64(x)
         g = np.zeros(self.ndens)
