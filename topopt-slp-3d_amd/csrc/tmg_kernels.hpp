@@ -126,7 +126,12 @@ enum {
 enum {
     EPI_RESIDR = 4,  // out = r - A w
     EPI_POSTR = 5,   // out = w + invD (r - A w);   partial r . out
-    EPI_DIRDOT = 6   // out = A w, dnew = w;         partial w . out
+    EPI_DIRDOT = 6,  // out = A w, dnew = w;         partial w . out
+    // out = A w at every DOF, fixed ones included (no free-DOF mask);
+    // partial w . out. For the PCG's q = A d when its consumer (k_r_update)
+    // drops q at the fixed DOFs; w = d is 0 there, so w . out is unchanged.
+    // Spectral LDS kernel (k_ebe_lds) only.
+    EPI_DOTU = 7
 };
 
 template <typename T, typename TS>
@@ -904,7 +909,8 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
 // and the staging of plane L + 2 share it (double-buffered). Requires every
 // vector to be smaller than 4 GiB (checked by the host). Jacobi-type
 // epilogues (JACOBI, POSTR) do not read the free-DOF mask: at fixed DOFs
-// invD = 0 and the source value is 0, so the sweep writes 0 there.
+// invD = 0 and the source value is 0, so the sweep writes 0 there; DOTU
+// leaves the masking to its consumer.
 // WPE: waves per SIMD the register allocation must allow (1 = no limit).
 // ---------------------------------------------------------------------------
 typedef unsigned int bu2 __attribute__((__vector_size__(8)));    // operand types of the buffer builtins
@@ -1039,7 +1045,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const uint32_t zl = (uint32_t)min(L, nz - 1);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            e.m[c] = (EPI == EPI_JACOBI || EPI == EPI_POSTR) ? 1u
+            e.m[c] = (EPI == EPI_JACOBI || EPI == EPI_POSTR || EPI == EPI_DOTU) ? 1u
                      : __builtin_amdgcn_raw_buffer_load_b8(rm, own * 3u + c, zl * plane * 3u, 0);
         if (EPI == EPI_RESID || EPI == EPI_JACOBI) bld3(rb, own * 3u * (uint32_t)sizeof(T), zl * pT, e.b);
         if (EPI == EPI_JACOBI || EPI == EPI_POSTR) bld3(ri, own * 3u * (uint32_t)sizeof(T), zl * pT, e.inv);
@@ -1155,7 +1161,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
                     o[c] = m ? (ec.b[c] - y) : T(0);
                 } else if (EPI == EPI_JACOBI) {
                     o[c] = m ? (w + ec.inv[c] * (ec.b[c] - y)) : T(0);
-                } else if (EPI == EPI_DOT) {
+                } else if (EPI == EPI_DOT || EPI == EPI_DOTU) {
                     o[c] = m ? y : T(0);
                     dacc += (double)w * (double)o[c];
                 } else if (EPI == EPI_RESIDR) {
@@ -1186,7 +1192,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         step(L + 1, 0, PB, PA, wB, wA, EB, EA, epB, epA);
     }
     if (L < z1) step(L, 1, PA, PB, wA, wB, EA, EB, epA, epB);
-    if (EPI == EPI_DOT || EPI == EPI_POSTR) block_reduce_store(dacc, partial_, (int64_t)bz * gridDim.x + bt);
+    if (EPI == EPI_DOT || EPI == EPI_DOTU || EPI == EPI_POSTR) block_reduce_store(dacc, partial_, (int64_t)bz * gridDim.x + bt);
 }
 
 
@@ -1562,6 +1568,8 @@ __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict
 // (x_{k+1} = x_k + alpha_k d_k is the same sum, done when d_k is read anyway):
 // k_r_update: r -= alpha q; partial ||r||^2; and the next preconditioner
 // cycle's first Jacobi sweep from zero, x0 = invD r (in its precision TX).
+// invD is 0 exactly at the fixed DOFs (and positive elsewhere), so r stays 0
+// there whatever q holds: q may come from the unmasked apply (EPI_DOTU).
 template <typename TX>
 __global__ __launch_bounds__(TPB) void k_r_update(int64_t n, double *__restrict__ r, const double *__restrict__ q,
                                                   const TX *__restrict__ invD, TX *__restrict__ x0, double *partial,
@@ -1570,10 +1578,11 @@ __global__ __launch_bounds__(TPB) void k_r_update(int64_t n, double *__restrict_
     const double al = sc->alpha;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
-        const double ri = r[i] - al * q[i];
+        const TX iv = invD[i];
+        const double ri = iv != TX(0) ? r[i] - al * q[i] : 0.0;
         r[i] = ri;
         s += ri * ri;
-        x0[i] = invD[i] * (TX)ri;
+        x0[i] = iv * (TX)ri;
     }
     block_reduce_store(s, partial, blockIdx.x);
 }

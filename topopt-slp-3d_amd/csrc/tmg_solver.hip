@@ -511,6 +511,23 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
     }
 }
 
+// The PCG's q = A d with d'q. With the residual update deferred
+// (k_r_update drops q at the fixed DOFs) the LDS kernel skips the free-DOF
+// mask (EPI_DOTU); otherwise, or where that kernel does not serve the fine
+// level, the masked apply gives the same free-DOF values.
+void apply_pcg_q(tmg_solver *h, bool consumer_masks, const int *done) {
+    const DevLevel &l0 = h->lev[0];
+    if (consumer_masks && h->wht && h->ebe_lds && h->nsub == 1 && !h->fine_stencil) {
+        dim3 gw((unsigned)(h->wht_ntx * h->wht_nty), (unsigned)((l0.nz + h->zchunk64 - 1) / h->zchunk64));
+        hipLaunchKernelGGL((k_ebe_lds<double, EPI_DOTU, 1, 3>), gw, dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz,
+                           h->zchunk64, h->wht_tyl, h->wht_ntx, (const double *)h->pd, (const double *)h->E64, l0.mask,
+                           (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, h->pq, h->partial,
+                           done, (const double *)h->kw64_d, h->xcd_wht);
+        return;
+    }
+    apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);
+}
+
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
@@ -796,7 +813,7 @@ void pcg_iteration(tmg_solver *h, int parity) {
             launch_ebe<double, double, SRC_X, EPI_DOT>(h, a);
             kt_mark(h, KT_APPLY, true);
         } else {
-            apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, done);      // q = A d, d'q
+            apply_pcg_q(h, xd, done);      // q = A d, d'q
             kt_mark(h, KT_APPLY, true);
             hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr,
                                h->d_sc);
@@ -1838,7 +1855,7 @@ int tmg_time_kernel(tmg_solver *h, int which, int nrep, double *avg_ms) {
     HIPX(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, h->st));
     auto launch = [&]() {
         if (which == 0) {
-            apply<double, EPI_DOT>(h, 0, h->pd, nullptr, h->pq, h->partial, nullptr);
+            apply_pcg_q(h, xdefer_ok(h), nullptr);
         } else if (which == 1 || which == 2) {
             const int l = which == 1 ? 0 : 1;
             DevLevel &lv = h->lev[l];
