@@ -19,6 +19,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "topopt-slp-3d_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# Serving target, stated ahead of the measurement: p99 latency (arrival to
+# result) of the 256-request stream at 0.8 x single-stream capacity, about six
+# times the stream's mean serial service time (~16 ms per request)
+SERVE_P99_TARGET_MS = 100.0
 FP64_VEC_PEAK_TFLOPS = 78.6
 FP32_VEC_PEAK_TFLOPS = 78.6   # packed FP32 is disabled in this build; unpacked FP32 = FP64 vector rate
 
@@ -385,6 +389,8 @@ def main():
         reqs = serve.stream_workload(args.serve_requests, seed=4)
         serving = serve.benchmark(reqs, budget_bytes=200e9, max_active=16, cap_sample=32, load=0.8, log=log)
         serving = {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in serving.items()}
+        serving["p99_target_ms"] = SERVE_P99_TARGET_MS
+        serving["p99_target_met"] = bool(serving["p99_ms"] <= SERVE_P99_TARGET_MS)
         log("serving:", json.dumps(serving))
     cpu = None
     if not args.no_cpu:
@@ -416,11 +422,15 @@ def main():
             "preconditioner_sweep": secondary,
             "serving": serving,
             "served_p99_ms": serving["p99_ms"] if serving else None,
+            "served_p99_target_ms": SERVE_P99_TARGET_MS,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if serving and serving["failed"]:
+        log(f"serving: {serving['failed']} request(s) failed: {serving.get('failed_requests')}")
+        sys.exit(3)
 
 
 if __name__ == "__main__":
