@@ -1,0 +1,92 @@
+"""The serving stream and the multiresolution SLP workload at their full
+sizes. Serving: the 256-request stream of 32x16x16 ... 128x64x64 grids
+(Poisson arrivals, 16 slots, 200 GB budget, mixed precision, tol 1e-8).
+Multiresolution: the 240x120x120 cantilever with 2x2x2 density sub-elements
+(10.5M DOFs, 27.6M design variables, 4 levels), where the oracle is too slow
+to run, checked through properties that hold at any size."""
+import sys
+
+import numpy as np
+import pytest
+
+from tmg import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def test_serving_stream_256_requests(oracle_mod):
+    """All 256 requests of the benchmark's stream converge to 1e-8 under the
+    200 GB budget with requests overlapping on the device; the smallest
+    requests' results meet the oracle (true residual, compliance)."""
+    from tmg import serve
+    reqs = serve.stream_workload(256, seed=4)
+    srv = serve.Server(200e9, max_active=16)
+    _say("serving stream: 256 requests at 40 requests/s")
+    recs = serve.run_stream(srv, reqs, rate=40.0, seed=4)
+    st = srv.stats()
+    srv.close()
+    summ = serve.summarize(recs)
+    _say(f"  p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms, {summ['requests_per_s']:.1f} requests/s, "
+         f"peak running {st['peak_running']}, peak reserved {st['peak_bytes_reserved'] / 1e9:.2f} GB")
+    assert summ["failed"] == 0 and summ["all_converged"]
+    assert st["peak_bytes_reserved"] <= 200e9 and st["peak_running"] >= 2
+    small = [i for i, r in enumerate(reqs) if r["prob"].nelx == 32][:3]
+    assert small
+    for i in small:
+        r, m = reqs[i], recs[i]
+        prob = r["prob"]
+        o = oracle_mod.Oracle(prob)
+        o.assemble(r["x"])
+        o.setup_gmg(r["ngrids"], 0.6)
+        u_or, it_or, _ = o.solve(prob.f)
+        relres = np.linalg.norm(prob.f - o.spmv(0, m["u"])) / np.linalg.norm(prob.f)
+        C, Co = prob.f @ m["u"], prob.f @ u_or
+        _say(f"  request {i}: it {m['it']} (oracle {it_or}), true relres {relres:.2e}, "
+             f"compliance err {abs(C - Co) / abs(Co):.2e}")
+        assert relres <= 1.2e-8 and abs(C - Co) / abs(Co) < 1e-7
+
+
+def test_multiresolution_240_uniform_density_scaling():
+    """For uniform sub-densities K(rho) = E(rho) K(1), so the compliance
+    C(rho) E(rho) is the same for every rho: rho = 1 and rho = 0.3 at
+    240x120x120 in mixed precision, both solves to 1e-8, agree to 1e-7; the
+    FP64 and mixed solves of rho = 0.3 agree to 1e-7."""
+    import tmg
+    prob = P.cantilever(240, 120, 120)
+    nd = 8 * prob.nelem
+    vals = {}
+    for prec in ("mixed", "fp64"):
+        s = tmg.Solver(prob, 4, precision=prec, mr=2)
+        for rho in ((1.0, 0.3) if prec == "mixed" else (0.3,)):
+            s.update_density(np.full(nd, rho))
+            u, it, rr, rc = s.solve(prob.f)
+            assert rc == 0 and rr <= 1e-8
+            E = 1e-9 + rho ** 3 * (1.0 - 1e-9)
+            vals[(prec, rho)] = float(prob.f @ u) * E
+            _say(f"  {prec} rho {rho}: it {it}, C*E {vals[(prec, rho)]:.12e}")
+        s.close()
+    ref = vals[("mixed", 1.0)]
+    assert abs(vals[("mixed", 0.3)] - ref) <= 1e-7 * abs(ref)
+    assert abs(vals[("fp64", 0.3)] - vals[("mixed", 0.3)]) <= 1e-7 * abs(ref)
+
+
+def test_multiresolution_240_slp_two_iterations():
+    """Two accepted SLP iterations of the multiresolution workload (volfrac
+    0.3 as an inequality, linear filter rmin 1.5, W-cycle, mixed precision):
+    every solve converges, the volume row holds, the designs stay in [0, 1],
+    and the objective decreases from one accepted step to the next."""
+    from tmg import slp
+    prob = P.cantilever(240, 120, 120)
+    res = slp.StrSLP(prob, 0.3, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, ngrids=4, precision="mixed", mr=2,
+                     slp=slp.SLPParams(maxiter=2), log=_say, cycle=1)
+    F = [h["F"] for h in res.history if h["accepted"]]
+    _say(f"  iterations {res.iter}, rejected {res.itrej}, pcg {res.pcgit}, F {F}, vol {res.vol:.12f}")
+    assert res.iter == 2 and len(F) == 2 and all(it < 5000 for it in res.pcgit)
+    assert 0.29 <= res.vol <= 0.3 + 1e-9
+    assert res.xstarDsgn.min() >= 0.0 and res.xstarDsgn.max() <= 1.0
+    assert all(np.isfinite(F)) and F[1] < F[0]
