@@ -905,8 +905,11 @@ __global__ __launch_bounds__(TPB) void k_ebe_wht(int nelx_, int nely_, int nelz_
 // (no per-step 64-bit address arithmetic; out-of-range buffer accesses read 0
 // and are dropped, so every access stays inside its buffer). The next plane,
 // the next layer's moduli and the next output node's epilogue operands are
-// loaded one step ahead. One barrier per step: the corner exchange of plane L
-// and the staging of plane L + 2 share it (double-buffered). Requires every
+// loaded one step ahead. One barrier per step: the exchange of plane L
+// between neighbouring element columns and the staging of plane L + 2 share it
+// (double-buffered). The exchange is separable: the y neighbour's part moves
+// one lane (DPP), the x neighbour's part (one value per component) goes
+// through LDS. Requires every
 // vector to be smaller than 4 GiB (checked by the host). Jacobi-type
 // epilogues (JACOBI, POSTR) do not read the free-DOF mask: at fixed DOFs
 // invD = 0 and the source value is 0, so the sweep writes 0 there; DOTU
@@ -948,6 +951,17 @@ __device__ inline double bld1(brsrc_t r, uint32_t v, uint32_t s, double) {
 }
 __device__ inline float bld1(brsrc_t r, uint32_t v, uint32_t s, float) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+}
+
+// Value of lane l - 1 (0 in lane 0), DPP wave_shr:1
+__device__ inline float wave_shr1(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
+}
+__device__ inline double wave_shr1(double v) {
+    int2 iv = __builtin_bit_cast(int2, v);
+    iv.x = __builtin_amdgcn_update_dpp(0, iv.x, 0x138, 0xF, 0xF, true);
+    iv.y = __builtin_amdgcn_update_dpp(0, iv.y, 0x138, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, iv);
 }
 
 constexpr int LDS_REGION_MAX = 300;   // (TX + 1)(TY + 1) for TY in {8, 16, 32}: at most 297
@@ -1007,7 +1021,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const uint32_t sEz = (uint32_t)nelx * nely * NSUB * (uint32_t)sizeof(T);
 
     __shared__ T lin[2][3][LDS_REGION_MAX];
-    __shared__ T xch[2][3][3][TPB];
+    __shared__ T xch[2][3][TPB];   // [buffer][component][thread]: x-stage values for column tx + 1
+    const int xq = max(t - TY, 0);
 
     T R0[3], R1[3];
     auto load_raw = [&](int zz) {
@@ -1129,7 +1144,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
             }
         }
         asm volatile("" : "+s"(kp) : "v"(yh[2][7]));
-        T cv[3][4];
+        // node plane L: S = this layer's bottom spectra plus the previous
+        // layer's top ones; the inverse 2x2 transform and the sum over the
+        // four element columns around a node are done separably: in y (S
+        // index k = ky + 2 kx) with a one-lane wave shift, since t - 1 is
+        // column (tx, ty - 1) whenever ty >= 1, then in x through LDS
+        T C[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             T S[4];
@@ -1138,12 +1158,11 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
                 S[k] = Qtp[c][k] + (yh[c][k] + yh[c][k + 4]);
                 Qtp[c][k] = yh[c][k] - yh[c][k + 4];
             }
-            wht4(S, cv[c]);
+            const T A0 = S[0] + S[1], B0 = S[0] - S[1], A1 = S[2] + S[3], B1 = S[2] - S[3];
+            const T U0 = A0 + wave_shr1(B0), U1 = A1 + wave_shr1(B1);
+            C[c] = U0 + U1;
+            xch[p][c][t] = U0 - U1;
         }
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) xch[p][j][c][t] = cv[c][j + 1];
         store_lin(p ^ 1);          // plane L + 2
         load_raw(L + 3);
         load_epi(L + 1, en);
@@ -1152,7 +1171,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
             T o[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T y = cv[c][0] + xch[p][0][c][t - 1] + xch[p][1][c][t - TY] + xch[p][2][c][t - TY - 1];
+                const T y = C[c] + xch[p][c][xq];
                 const bool m = ec.m[c] != 0;
                 const T w = wb[c];
                 if (EPI == EPI_PLAIN) {
