@@ -4,16 +4,18 @@
 //
 // Requests are admitted FIFO while a slot is free and the estimated device
 // memory of the request fits the budget. Each slot owns a HIP stream, a
-// pinned scalar block, a pinned staging buffer for the request's host vectors
-// and a stream-ordered memory pool that never hands memory to another stream,
-// so admitting and retiring requests never synchronises the device and no
-// request can be given memory another request's kernels still use. A setup
-// thread builds each admitted hierarchy (Galerkin refresh, coarse factor,
-// graph capture) and checks that the device holds the request's densities;
-// a driver thread keeps every running solve supplied with chunks of PCG
-// iterations; kernels of different requests overlap on the device. Results
-// are bitwise identical to a solve run alone: every kernel is deterministic
-// and a request's launches depend only on its own problem.
+// pinned scalar block, a pinned staging buffer for the request's host vectors,
+// a rocBLAS handle and a stream-ordered memory pool that never hands memory to
+// another stream, so admitting and retiring requests never synchronises the
+// device and no request can be given memory another request's kernels still
+// use. Each slot is served by its own worker thread: it builds the admitted
+// request's hierarchy (Galerkin refresh, coarse factor, graph capture), checks
+// that the device holds the request's densities, then drives the solve with
+// graph launches of growing chunks of PCG iterations, waiting on its own
+// stream only. The workers' host-side launch work runs in parallel and the
+// kernels of different requests overlap on the device. Results are bitwise
+// identical to a solve run alone: every kernel is deterministic and a
+// request's launches depend only on its own problem.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -23,7 +25,9 @@
 
 namespace {
 
-constexpr int SRV_SETUP_THREADS = 4;
+// headroom of the admission estimate for what the handle does not allocate
+// itself (rocSOLVER's workspace): not part of the handle's arena
+constexpr size_t SRV_ARENA_SLACK = (size_t)32 << 20;
 
 struct SrvReq {
     int64_t id = 0;
@@ -35,12 +39,12 @@ struct SrvReq {
     size_t est = 0;
     int slot = -1;
     int state = 0;   // 0 queued, 1 setting up, 2 running, 3 done, < 0 failed (error code)
-    int chunk = 1;
-    bool in_flight = false;
     tmg_solver *h = nullptr;
     std::vector<double> u;
     int it = 0, rc = 0;
     double relres = 0.0;
+    char *arena = nullptr;    // device block of the slot's pool holding the handle's buffers
+    size_t arena_bytes = 0;
     double t_arrival = 0, t_admit = 0, t_start = 0, t_done = 0;
     std::string err;
 };
@@ -95,7 +99,7 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
     const int64_t nblocks = std::max<int64_t>(pb * nn0z, 4096);
     add(8 * nblocks); add(8 * nblocks); add(sizeof(PcgScalars));
     add(8 * nfree);                                            // upload staging
-    return b + (32u << 20);                                    // rocSOLVER workspace headroom
+    return b + SRV_ARENA_SLACK;                                // rocSOLVER workspace headroom
 }
 
 }  // namespace
@@ -122,13 +126,10 @@ struct tmg_server {
     std::unordered_map<int64_t, SrvReq *> all;
     int64_t next_id = 1;
     bool stop = false;
-    // several setup threads build admitted hierarchies concurrently (host work,
-    // graph capture and the coarse factorization of one request overlap with
-    // the others'); each owns a rocBLAS handle, whose stream binding is
-    // per handle
-    std::vector<std::thread> setup_thr;
+    // one worker thread per slot (set-up, solve and retirement of the slot's
+    // requests); each owns a rocBLAS handle, whose stream binding is per handle
+    std::vector<std::thread> workers;
     std::vector<rocblas_handle> rbs;
-    std::thread drive_thr;
     std::chrono::steady_clock::time_point t0;
     std::string err;
     double now() const {
@@ -185,143 +186,151 @@ void srv_diagnose_not_spd(tmg_server *S, tmg_solver *h, SrvReq *r) {
              (rc2 == 0 ? "factorised (the first build was corrupted)" : "failed again: " + h->err);
 }
 
-void srv_setup_loop(tmg_server *S, rocblas_handle rb) {
+// The request's arena goes back to its slot's pool, ordered after the
+// request's work on the slot's stream.
+void srv_free_arena(tmg_server *S, SrvReq *r) {
+    if (r->arena) (void)hipFreeAsync(r->arena, S->streams[r->slot]);
+    r->arena = nullptr;
+    r->arena_bytes = 0;
+}
+
+// Set-up of an admitted request on its slot: hierarchy, densities check,
+// host vectors, PCG start. Returns the handle (nullptr and *rc < 0 on failure).
+tmg_solver *srv_setup(tmg_server *S, SrvReq *r, rocblas_handle rb, int *rc, std::string *msg) {
+    const int slot = r->slot;
+    *rc = 0;
+    // pinned staging large enough for the densities and the free vectors
+    const size_t need = std::max(r->x.size(), r->f.size());
+    if (S->stage_cap[slot] < need) {
+        if (S->stage[slot]) (void)hipHostFree(S->stage[slot]);
+        S->stage[slot] = nullptr;
+        S->stage_cap[slot] = 0;
+        const size_t cap = need + need / 4;
+        if (hipHostMalloc((void **)&S->stage[slot], cap * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            S->stage[slot] = nullptr;
+            *rc = TMG_ERR_OOM;
+            *msg = "pinned staging allocation failed";
+            return nullptr;
+        }
+        S->stage_cap[slot] = cap;
+    }
+    // one block of the slot's pool for all of the handle's buffers (the
+    // estimate bounds them; the handle falls back to the pool beyond it)
+    r->arena_bytes = r->est > SRV_ARENA_SLACK ? r->est - SRV_ARENA_SLACK : r->est;
+    if (hipMallocFromPoolAsync((void **)&r->arena, r->arena_bytes, S->pools[slot], S->streams[slot]) != hipSuccess) {
+        r->arena = nullptr;
+        r->arena_bytes = 0;
+    }
+    ExtRes ext;
+    ext.st = S->streams[slot];
+    ext.rb = rb;
+    ext.h_sc = S->hsc[slot];
+    ext.pool = S->pools[slot];
+    ext.arena = r->arena;
+    ext.arena_cap = r->arena_bytes;
+    ext.stage = S->stage[slot];
+    ext.stage_cap = S->stage_cap[slot];
+    r->prob.freedofs = r->freedofs.data();
+    tmg_solver *h = nullptr;
+    *rc = setup_impl(&r->prob, &r->mg, r->prec, &ext, &h);
+    if (*rc) {
+        *msg = g_setup_err;
+        srv_free_arena(S, r);
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> fk(S->factor_mu);
+        *rc = tmg_update_density(h, r->x.data());
+    }
+    if (!*rc) *rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
+    if (*rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
+    if (!*rc && (*rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) *rc = upload_free(h, 0, r->f.data(), h->pf);
+    h->have_rhs = true;
+    if (!*rc) *rc = solve_begin(h);
+    if (*rc) {
+        *msg = h->err;
+        destroy_all(h);
+        delete h;
+        srv_free_arena(S, r);
+        return nullptr;
+    }
+    return h;
+}
+
+// The PCG of a running request in graph launches of 1, 2, 4, 8, 8, ... chunks
+// (a chunk is the captured run of iterations; iterations past convergence
+// return at once on the device), with the mixed-precision stall check between
+// launches; then the deferred last update and the download of u.
+int srv_solve(SrvReq *r, tmg_solver *h) {
+    int chunk = 1;
+    for (;;) {
+        TRY(solve_launch(h, chunk));
+        HIPX(hipEventSynchronize(h->ev));
+        if (h->h_sc->done) break;
+        TRY(solve_check(h));
+        chunk = std::min(chunk * 2, 8);
+    }
+    int rc = solve_result(h, &r->it, &r->relres);
+    if (rc >= 0) rc = std::min(rc, solve_finish(h));
+    if (rc >= 0) {
+        r->u.assign(h->lev[0].nfree, 0.0);
+        const int d = download_free(h, 0, h->px, r->u.data());
+        if (d < 0) rc = d;
+    }
+    return rc;
+}
+
+// Worker of one slot: admits the queue's head when the budget allows (FIFO),
+// sets it up, solves it and retires it.
+void srv_worker(tmg_server *S, int slot) {
     (void)hipSetDevice(0);
+    rocblas_handle rb = S->rbs[slot];
     for (;;) {
         SrvReq *r = nullptr;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv.wait(lk, [&] {
                 if (S->stop) return true;
-                if (S->queue.empty()) return false;
-                SrvReq *q = S->queue.front();
-                bool slot = false;
-                for (int i = 0; i < S->nslots; ++i) slot |= !S->slot_busy[i];
-                return slot && S->used + q->est <= S->budget;
+                return !S->queue.empty() && S->used + S->queue.front()->est <= S->budget;
             });
             if (S->stop) return;
             r = S->queue.front();
             S->queue.pop_front();
-            for (int i = 0; i < S->nslots; ++i)
-                if (!S->slot_busy[i]) { r->slot = i; S->slot_busy[i] = 1; break; }
+            r->slot = slot;
+            S->slot_busy[slot] = 1;
             S->used += r->est;
             S->peak = std::max(S->peak, S->used);
             r->state = 1;
             r->t_admit = S->now();
         }
-        const int slot = r->slot;
         int rc = 0;
         std::string msg;
-        // pinned staging large enough for the densities and the free vectors
-        const size_t need = std::max(r->x.size(), r->f.size());
-        if (S->stage_cap[slot] < need) {
-            if (S->stage[slot]) (void)hipHostFree(S->stage[slot]);
-            S->stage[slot] = nullptr;
-            S->stage_cap[slot] = 0;
-            const size_t cap = need + need / 4;
-            if (hipHostMalloc((void **)&S->stage[slot], cap * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-                S->stage[slot] = nullptr;
-                rc = TMG_ERR_OOM;
-                msg = "pinned staging allocation failed";
-            } else {
-                S->stage_cap[slot] = cap;
-            }
-        }
-        ExtRes ext;
-        ext.st = S->streams[slot];
-        ext.rb = rb;
-        ext.h_sc = S->hsc[slot];
-        ext.pool = S->pools[slot];
-        ext.stage = S->stage[slot];
-        ext.stage_cap = S->stage_cap[slot];
-        r->prob.freedofs = r->freedofs.data();
-        tmg_solver *h = nullptr;
-        if (!rc) {
-            rc = setup_impl(&r->prob, &r->mg, r->prec, &ext, &h);
-            if (rc) msg = g_setup_err;
-        }
-        if (!rc) {
-            {
-                std::lock_guard<std::mutex> fk(S->factor_mu);
-                rc = tmg_update_density(h, r->x.data());
-            }
-            if (!rc) rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
-            if (rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
-            if (!rc && (rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) rc = upload_free(h, 0, r->f.data(), h->pf);
-            h->have_rhs = true;
-            if (!rc) rc = solve_begin(h);
-            if (rc) msg = h->err;
-        }
-        std::lock_guard<std::mutex> lk(S->mu);
-        if (rc) {
-            if (h) { destroy_all(h); delete h; }
-            srv_release(S, r);
-            srv_fail(S, r, rc, msg);
-        } else {
+        tmg_solver *h = srv_setup(S, r, rb, &rc, &msg);
+        if (h) {
+            std::lock_guard<std::mutex> lk(S->mu);
             r->h = h;
             r->state = 2;
             r->t_start = S->now();
             S->running.push_back(r);
             S->peak_running = std::max(S->peak_running, (int)S->running.size());
         }
-        S->cv.notify_all();
-    }
-}
-
-void srv_drive_loop(tmg_server *S) {
-    (void)hipSetDevice(0);
-    std::vector<SrvReq *> act;
-    for (;;) {
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            if (S->stop) return;
-            act = S->running;
-        }
-        bool progressed = false;
-        std::vector<SrvReq *> finished;
-        for (SrvReq *r : act) {
-            tmg_solver *h = r->h;
-            if (r->in_flight) {
-                int p = solve_poll(h);
-                if (p == 0) continue;
-                r->in_flight = false;
-                progressed = true;
-                if (p < 0) { r->rc = p; finished.push_back(r); continue; }
-                if (h->h_sc->done) {
-                    r->rc = solve_result(h, &r->it, &r->relres);
-                    finished.push_back(r);
-                    continue;
-                }
-                const int sck = solve_check(h);
-                if (sck < 0) { r->rc = sck; finished.push_back(r); continue; }
-            }
-            int rc = solve_launch(h, r->chunk);
-            r->chunk = std::min(r->chunk * 2, 4);
-            if (rc < 0) { r->rc = rc; finished.push_back(r); continue; }
-            r->in_flight = true;
-            progressed = true;
-        }
-        for (SrvReq *r : finished) {
-            tmg_solver *h = r->h;
-            std::string msg = h->err;
-            int rc = r->rc;
-            if (rc >= 0) rc = std::min(rc, solve_finish(h));
-            if (rc >= 0) {
-                r->u.assign(h->lev[0].nfree, 0.0);
-                int d = download_free(h, 0, h->px, r->u.data());
-                if (d < 0) { rc = d; msg = h->err; }
-            }
+        if (h) {
+            rc = srv_solve(r, h);
+            r->rc = rc;
+            if (rc < 0) msg = h->err;
             destroy_all(h);
             delete h;
-            std::lock_guard<std::mutex> lk(S->mu);
+            srv_free_arena(S, r);
+        }
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (h) {
             r->h = nullptr;
             S->running.erase(std::find(S->running.begin(), S->running.end(), r));
-            srv_release(S, r);
-            if (rc < 0) srv_fail(S, r, rc, msg);
-            else { r->state = 3; r->t_done = S->now(); }
-            S->cv.notify_all();
         }
-        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        srv_release(S, r);
+        if (rc < 0) srv_fail(S, r, rc, msg);
+        else { r->state = 3; r->t_done = S->now(); }
+        S->cv.notify_all();
     }
 }
 
@@ -373,14 +382,12 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
     S->stage.assign(max_active, nullptr);
     S->stage_cap.assign(max_active, 0);
     S->slot_busy.assign(max_active, 0);
-    const int nsetup = std::min(max_active, SRV_SETUP_THREADS);
-    for (int i = 0; i < nsetup; ++i) {
+    for (int i = 0; i < max_active; ++i) {
         rocblas_handle h = nullptr;
         if (rocblas_create_handle(&h) != rocblas_status_success) return bad("rocblas_create_handle failed");
         S->rbs.push_back(h);
     }
-    for (int i = 0; i < nsetup; ++i) S->setup_thr.emplace_back(srv_setup_loop, S, S->rbs[i]);
-    S->drive_thr = std::thread(srv_drive_loop, S);
+    for (int i = 0; i < max_active; ++i) S->workers.emplace_back(srv_worker, S, i);
     *out = S;
     return TMG_OK;
 }
@@ -489,9 +496,8 @@ void tmg_server_destroy(tmg_server *S) {
         S->stop = true;
     }
     S->cv.notify_all();
-    for (std::thread &t : S->setup_thr)
+    for (std::thread &t : S->workers)
         if (t.joinable()) t.join();
-    if (S->drive_thr.joinable()) S->drive_thr.join();
     for (SrvReq *r : S->running) {
         if (r->h) { destroy_all(r->h); delete r->h; }
     }

@@ -153,6 +153,12 @@ struct tmg_solver {
     // stream-ordered pool, and the stream, rocBLAS handle and pinned scalar
     // block are lent, so nothing here synchronises the whole device
     hipMemPool_t pool = nullptr;
+    // or from one lent block (the server's per-request arena): allocations are
+    // carved from it in order and a free of the most recent one gives its
+    // space back (the temporaries of set-up and of the host-vector transfers
+    // are freed in that order); the block itself belongs to the lender
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
     double *stage = nullptr;          // pinned host staging lent by a server slot: host vectors pass through it
     size_t stage_cap = 0;             // its capacity in doubles
     bool own_stream = true, own_rb = true, own_hsc = true;
@@ -215,7 +221,13 @@ bool poison_buffers() {
 template <typename T>
 int dalloc(tmg_solver *h, T **p, int64_t n) {
     const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
-    hipError_t e = h->pool ? hipMallocFromPoolAsync((void **)p, bytes, h->pool, h->st) : hipMalloc((void **)p, bytes);
+    hipError_t e = hipSuccess;
+    if (h->arena && h->arena_off + round256(bytes) <= h->arena_cap) {
+        *p = reinterpret_cast<T *>(h->arena + h->arena_off);
+        h->arena_off += round256(bytes);
+    } else {
+        e = h->pool ? hipMallocFromPoolAsync((void **)p, bytes, h->pool, h->st) : hipMalloc((void **)p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         h->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
@@ -231,9 +243,16 @@ int dalloc(tmg_solver *h, T **p, int64_t n) {
 template <typename T>
 void dfree(tmg_solver *h, T *&p) {
     if (p) {
-        if (h->pool) (void)hipFreeAsync((void *)p, h->st);
-        else (void)hipFree((void *)p);
         auto it = h->sizes.find((void *)p);
+        const size_t sz = it != h->sizes.end() ? it->second : 0;
+        char *c = reinterpret_cast<char *>(p);
+        if (h->arena && c >= h->arena && c < h->arena + h->arena_cap) {
+            if (sz && c + sz == h->arena + h->arena_off) h->arena_off -= sz;   // the most recent block
+        } else if (h->pool) {
+            (void)hipFreeAsync((void *)p, h->st);
+        } else {
+            (void)hipFree((void *)p);
+        }
         if (it != h->sizes.end()) {
             h->bytes -= it->second;
             h->sizes.erase(it);
@@ -1288,6 +1307,8 @@ struct ExtRes {
     rocblas_handle rb = nullptr;
     PcgScalars *h_sc = nullptr;
     hipMemPool_t pool = nullptr;
+    char *arena = nullptr;            // device block the handle's buffers are carved from (optional)
+    size_t arena_cap = 0;
     double *stage = nullptr;
     size_t stage_cap = 0;
 };
@@ -1359,6 +1380,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         h->rb = ext->rb;
         h->h_sc = ext->h_sc;
         h->pool = ext->pool;
+        h->arena = ext->arena;
+        h->arena_cap = ext->arena ? ext->arena_cap : 0;
         h->stage = ext->stage;
         h->stage_cap = ext->stage_cap;
         h->own_stream = h->own_rb = h->own_hsc = false;
