@@ -117,8 +117,6 @@ struct tmg_server {
     std::vector<size_t> stage_cap;
     std::vector<hipMemPool_t> pools;
     std::vector<char> slot_busy;
-    // the coarse factorisations of concurrent set-ups run one at a time
-    std::mutex factor_mu;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<SrvReq *> queue;
@@ -126,6 +124,7 @@ struct tmg_server {
     std::unordered_map<int64_t, SrvReq *> all;
     int64_t next_id = 1;
     bool stop = false;
+    bool trace = false;            // TMG_SERVER_TRACE=1: per-request set-up phase times on stderr
     // one worker thread per slot (set-up, solve and retirement of the slot's
     // requests); each owns a rocBLAS handle, whose stream binding is per handle
     std::vector<std::thread> workers;
@@ -177,11 +176,7 @@ void srv_diagnose_not_spd(tmg_server *S, tmg_solver *h, SrvReq *r) {
     std::string dens = device_checksum(h, h->xdens, (int64_t)r->x.size(), &dev) == 0
                            ? (dev == host_checksum(r->x.data(), (int64_t)r->x.size()) ? "intact" : "corrupted")
                            : "unreadable";
-    int rc2;
-    {
-        std::lock_guard<std::mutex> fk(S->factor_mu);
-        rc2 = refresh(h);
-    }
+    const int rc2 = refresh(h);
     h->err = first + "; densities on the device " + dens + "; rebuilding the same hierarchy " +
              (rc2 == 0 ? "factorised (the first build was corrupted)" : "failed again: " + h->err);
 }
@@ -232,21 +227,26 @@ tmg_solver *srv_setup(tmg_server *S, SrvReq *r, rocblas_handle rb, int *rc, std:
     ext.stage_cap = S->stage_cap[slot];
     r->prob.freedofs = r->freedofs.data();
     tmg_solver *h = nullptr;
+    const double tp0 = S->now();
     *rc = setup_impl(&r->prob, &r->mg, r->prec, &ext, &h);
+    const double tp1 = S->now();
     if (*rc) {
         *msg = g_setup_err;
         srv_free_arena(S, r);
         return nullptr;
     }
-    {
-        std::lock_guard<std::mutex> fk(S->factor_mu);
-        *rc = tmg_update_density(h, r->x.data());
-    }
+    *rc = tmg_update_density(h, r->x.data());   // each worker's own rocBLAS handle and stream
+    const double tp2 = S->now();
     if (!*rc) *rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
     if (*rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
     if (!*rc && (*rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) *rc = upload_free(h, 0, r->f.data(), h->pf);
     h->have_rhs = true;
+    const double tp3 = S->now();
     if (!*rc) *rc = solve_begin(h);
+    if (S->trace)
+        fprintf(stderr, "tmg_server: request %lld %dx%dx%d slot %d: structure %.2f ms, operator %.2f ms, vectors %.2f ms, "
+                "PCG start %.2f ms\n", (long long)r->id, r->prob.nelx, r->prob.nely, r->prob.nelz, slot,
+                1e3 * (tp1 - tp0), 1e3 * (tp2 - tp1), 1e3 * (tp3 - tp2), 1e3 * (S->now() - tp3));
     if (*rc) {
         *msg = h->err;
         destroy_all(h);
@@ -348,6 +348,7 @@ int tmg_server_create(double budget_bytes, int max_active, tmg_server **out) {
     tmg_server *S = new tmg_server();
     S->budget = (size_t)budget_bytes;
     S->nslots = max_active;
+    S->trace = getenv_flag("TMG_SERVER_TRACE");
     S->t0 = std::chrono::steady_clock::now();
     auto bad = [&](const std::string &m) {
         g_setup_err = m;
