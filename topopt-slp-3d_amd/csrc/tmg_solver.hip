@@ -966,16 +966,6 @@ int solve_check(tmg_solver *h) {
     return check_launch(h);
 }
 
-// 1 when the last launched chunk has completed (h_sc is then current), 0 if
-// it is still running, < 0 on error.
-int solve_poll(tmg_solver *h) {
-    hipError_t e = hipEventQuery(h->ev);
-    if (e == hipSuccess) return 1;
-    if (e == hipErrorNotReady) return 0;
-    h->err = std::string("hipEventQuery: ") + hipGetErrorString(e);
-    return TMG_ERR_HIP;
-}
-
 int solve_result(tmg_solver *h, int *it, double *relres) {
     *it = h->h_sc->it;
     *relres = h->h_sc->nr;
