@@ -357,7 +357,7 @@ def main():
     t = k["fine_apply_fp64_in_solve_ms"] * 1e-3
     achieved = bytes_apply64 / t / 1e9
     roofline = {
-        "bound": "hbm", "kernel": "k_ebe_lds<double,DOT> (fine-level FP64 operator apply q = K d with d'q, spectral element form)",
+        "bound": "hbm", "kernel": "k_ebe_lds<double,DOTU> (fine-level FP64 operator apply q = K d with d'q, spectral element form, separable node exchange)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": profile_traffic("k_ebe_lds_apply_fp64_cantilever192x96x96_hbm_traffic.txt"),
