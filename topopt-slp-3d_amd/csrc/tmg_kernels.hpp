@@ -1506,13 +1506,18 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(int fnx, int fny, int fnz, 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) acc[c] += w * ec[3 * J + c];
             }
+    // the node's three values are read and written as one 3-vector (one
+    // vector access per node instead of three strided ones)
     const int64_t i = (int64_t)iz * fnx * fny + p;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const int64_t dof = 3 * i + c;
-        const T v = fmask[dof] ? acc[c] : T(0);
-        xf[dof] = add ? xf[dof] + v : v;
+    T *xn = xf + 3 * i;
+    T xo[3] = {T(0), T(0), T(0)};
+    if (add) {
+        xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
     }
+    T xv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xv[c] = xo[c] + (fmask[3 * i + c] ? acc[c] : T(0));
+    xn[0] = xv[0]; xn[1] = xv[1]; xn[2] = xv[2];
 }
 
 // ---------------------------------------------------------------------------

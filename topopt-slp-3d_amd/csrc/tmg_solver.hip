@@ -506,7 +506,7 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
         constexpr bool PF8 = std::is_same<T, double>::value;
         dim3 gw((unsigned)(h->l1_ntx * h->l1_nty), (unsigned)((lv.nz + zc - 1) / zc));
         if constexpr (std::is_same<T, double>::value && EPI != EPI_DIRDOT && EPI != EPI_RESIDR) {
-            if (h->ebe_lds) {   // measured faster in FP64 only (FP32: the extra registers cost a wave)
+            if (h->ebe_lds) {   // measured faster in FP64; neutral in FP32 (k_ebe_wht kept there)
                 hipLaunchKernelGGL((k_ebe_lds<T, EPI, 8, 1>), gw, dim3(TPB), 0, h->st, lv.nelx, lv.nely, lv.nelz, zc,
                                    h->l1_tyl, h->l1_ntx, x, E1of<T>(h), lv.mask, b, invof<T>(lv),
                                    (const double *)nullptr, out, partial, done, KWGof<T>(h), h->xcd_wht);
