@@ -227,9 +227,13 @@ int64_t tmg_device_bytes(const tmg_solver *h);
  * inside its SLP loop): a continuous-batching server for streams of
  * equilibrium solves of uneven grid sizes. Requests are admitted FIFO as
  * slots and device memory (estimated per request, against hbm_budget_bytes)
- * free up; each admitted request gets its own device-resident hierarchy and
- * stream, and the PCG iterations of all running requests overlap on the
- * device. Each result is bitwise identical to the same solve run alone.
+ * free up; each of the max_active slots has its own worker thread, stream,
+ * memory pool and rocBLAS handle, each admitted request gets its own
+ * device-resident hierarchy (carved from one block of its slot's pool), and
+ * the PCG iterations of all running requests overlap on the device. Each
+ * result is bitwise identical to the same solve run alone. With the
+ * environment variable TMG_SERVER_TRACE=1 each request's set-up phase times
+ * are printed to stderr.
  * ------------------------------------------------------------------------- */
 typedef struct tmg_server tmg_server;
 
