@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-slp", action="store_true", help="skip the bounded sample of the multiresolution SLP loop")
+    ap.add_argument("--slp-sample-iters", type=int, default=2,
+                    help="SLP iterations of the multiresolution sample in the default run")
     ap.add_argument("--solve-only", default=None, choices=["fp64", "mixed"],
                     help="run only this solve (profiling aid; prints no JSON line)")
     ap.add_argument("--serve-requests", type=int, default=256)
@@ -233,6 +236,35 @@ def slp_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def slp_sample_leg(args):
+    """A bounded sample of the multiresolution SLP workload inside the default
+    run: the first args.slp_sample_iters accepted iterations of the 240x120x120
+    loop (2x2x2 density sub-elements, W-cycle, mixed precision) that
+    bench.py --workload slp runs for 30 iterations."""
+    from tmg import problem as P
+    from tmg import slp
+    nelx, nely, nelz = (int(v) for v in args.slp_grid.split("x"))
+    prob = P.cantilever(nelx, nely, nelz)
+    log(f"SLP sample: {nelx}x{nely}x{nelz} MR 2, {args.slp_sample_iters} iterations ...")
+    t0 = time.perf_counter()
+    res = slp.StrSLP(prob, 0.3, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, ngrids=4, precision="mixed", mr=2,
+                     slp=slp.SLPParams(maxiter=args.slp_sample_iters), log=lambda *a, **kw: None,
+                     cycle=args.slp_cycle)
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": f"cantilever {nelx}x{nely}x{nelz} Q1, multiresolution 2x2x2 density sub-elements "
+                    f"({8 * prob.nelem} design variables, {prob.nfree} free DOFs), volfrac 0.3, rmin 1.5, "
+                    f"4-level GMG {'VWF'[args.slp_cycle]}(1,1), tol 1e-8, mixed precision; the first "
+                    f"{args.slp_sample_iters} iterations of the loop bench.py --workload slp runs",
+        "iterations": res.iter, "rejected": res.itrej, "pcg_iterations": res.pcgit,
+        "unconverged_solves": int(sum(1 for it in res.pcgit if it > 5000)),
+        "F": res.F, "vol": res.vol, "wall_s": round(wall, 3),
+        "time_s": {kk: round(v, 3) for kk, v in res.time.items()},
+    }
+    log("SLP sample:", json.dumps(out))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -392,6 +424,9 @@ def main():
         serving["p99_target_ms"] = SERVE_P99_TARGET_MS
         serving["p99_target_met"] = bool(serving["p99_ms"] <= SERVE_P99_TARGET_MS)
         log("serving:", json.dumps(serving))
+    slp_sample = None
+    if not args.no_slp:
+        slp_sample = slp_sample_leg(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -423,6 +458,7 @@ def main():
             "serving": serving,
             "served_p99_ms": serving["p99_ms"] if serving else None,
             "served_p99_target_ms": SERVE_P99_TARGET_MS,
+            "slp_sample": slp_sample,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
