@@ -33,6 +33,8 @@ def test_serving_stream_256_requests(oracle_mod):
     summ = serve.summarize(recs)
     _say(f"  p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms, {summ['requests_per_s']:.1f} requests/s, "
          f"peak running {st['peak_running']}, peak reserved {st['peak_bytes_reserved'] / 1e9:.2f} GB")
+    for i, err in summ.get("failed_requests", []):
+        _say(f"  request {i} failed: {err}")
     assert summ["failed"] == 0 and summ["all_converged"]
     assert st["peak_bytes_reserved"] <= 200e9 and st["peak_running"] >= 2
     small = [i for i, r in enumerate(reqs) if r["prob"].nelx == 32][:3]

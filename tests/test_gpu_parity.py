@@ -435,21 +435,25 @@ def test_multiresolution_spectral_matches_stencil(monkeypatch, grid):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("nelx,nely,nelz,L,seed", [(16, 8, 8, 3, 21), (24, 12, 12, 3, 22)])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed,run", [(16, 8, 8, 3, 21, 8), (24, 12, 12, 3, 22, 8),
+                                                       (24, 12, 12, 3, 25, 2), (32, 16, 16, 3, 26, 3)])
 def test_symmetric_coarse_solve_matches_full_and_oracle(oracle_mod, monkeypatch, precision, nelx, nely, nelz, L,
-                                                        seed):
+                                                        seed, run):
     """The coarse solve that reads only the upper block triangle of the
-    symmetric inverse (k_coarse_symv1/2, the default from 16,384 coarse DOFs)
-    forced on small coarse systems whose size is not a multiple of the 64-row
-    block (108 and 288 DOFs): the V-cycle equals the full-matrix solve's to
-    rounding, and the PCG solve meets the oracle."""
-    _say(f"symmetric coarse solve {precision} {nelx}x{nely}x{nelz} L={L}")
+    symmetric inverse (k_coarse_symv_run/sum, the default from 16,384 coarse
+    DOFs) forced on small coarse systems whose size is not a multiple of the
+    64-row block (108, 288 and 600 DOFs), with runs of 8, 2 and 3 tiles (so
+    that block rows have several runs, a partial last run, and a last block
+    row and column that end inside a tile): the V-cycle equals the
+    full-matrix solve's to rounding, and the PCG solve meets the oracle."""
+    _say(f"symmetric coarse solve {precision} {nelx}x{nely}x{nelz} L={L} run={run}")
     prob = P.cantilever(nelx, nely, nelz)
     x = P.random_density(nelx, nely, nelz, seed)
     o = _oracle(oracle_mod, prob, x, L)
     monkeypatch.setenv("TMG_COARSE_SYMV", "0")
     s0 = _setup(prob, x, L, precision)
     monkeypatch.setenv("TMG_COARSE_SYMV", "1")
+    monkeypatch.setenv("TMG_CS_RUN", str(run))
     s1 = _setup(prob, x, L, precision)
     nc = s1.level_info(L)[1].size
     assert nc % 64 != 0

@@ -1739,77 +1739,146 @@ __global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const double *_
     }
 }
 
-// Coarse solve reading half of the symmetric inverse (large coarse systems):
-// 64x64 blocks (I, J), I <= J (pairs[p]); pass 1 forms u = A_IJ x_J and, off
-// the diagonal, v = A_IJ' x_I for each pair into P[p] = (u, v); pass 2 adds,
-// for block row I, the v of pairs (J, I), J < I, then the u of pairs (I, J),
-// J >= I, in that fixed order and scatters to the free DOFs. A is the full
-// (symmetric) n x n inverse; rows and columns past n read as zero.
+// Coarse solve reading half of the symmetric inverse (large coarse systems).
+// A (the full n x n inverse, row-major) is cut into 64x64 tiles (I, J); only
+// the tiles with J >= I are read, once each. xg holds b compacted to the free
+// DOFs (k_gather). Rows and columns past n read as zero.
+//
+// Pass 1, k_coarse_symv_run: one wave per run of up to RUN consecutive tiles
+// (I, Ja..Jb-1) of one block row. Lane c owns column c of every tile; the
+// wave reads the run in four 16-row groups, row by row (one 512-byte row
+// segment of a tile per load). Each lane accumulates the column products
+// v_J = A_IJ' x_I of its columns over the four groups (stored per tile pair,
+// J > I), and per group 16 partial row products A_I,J[r][c] x_J[c] summed
+// over the run's tiles; a transposing butterfly over lane bits 3..0 and two
+// shuffle sums over bits 4 and 5 leave the group's row products in lanes
+// 0..15.
+//
+// Pass 2, k_coarse_symv_sum: block row K sums the column products of the
+// tiles (I, K), I < K, then the row products of its own runs, each part split
+// over the block's 4 waves in contiguous ranges and the 4 wave sums added in
+// order; every sum has a fixed order.
 constexpr int CS_B = 64;
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_coarse_symv1(int64_t n, const double *__restrict__ A,
-                                                      const int2 *__restrict__ pairs,
-                                                      const int64_t *__restrict__ freeidx, const T *__restrict__ b,
-                                                      double *__restrict__ P, const int *done) {
-    if (done && *done) return;
-    const int2 pr = pairs[blockIdx.x];
-    const int64_t I0 = (int64_t)pr.x * CS_B, J0 = (int64_t)pr.y * CS_B;
-    const int c = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;   // column, group of 16 rows
-    __shared__ double xr[CS_B], xc[CS_B];
-    __shared__ double T_[CS_B][CS_B + 1];    // row products, padded
-    __shared__ double V[4][CS_B];
-    if (threadIdx.x < CS_B) {
-        const int64_t i = I0 + threadIdx.x;
-        xr[threadIdx.x] = i < n ? (double)b[freeidx[i]] : 0.0;
-    } else if (threadIdx.x < 2 * CS_B) {
-        const int64_t j = J0 + threadIdx.x - CS_B;
-        xc[threadIdx.x - CS_B] = j < n ? (double)b[freeidx[j]] : 0.0;
-    }
-    __syncthreads();
-    const int64_t col = J0 + c;
-    const bool cin = col < n;
-    const double xj = xc[c];
-    double v = 0.0;
-#pragma unroll 4
-    for (int rr = 0; rr < 16; ++rr) {
-        const int r = g * 16 + rr;
-        const int64_t row = I0 + r;
-        const double a = (cin && row < n) ? A[row * n + col] : 0.0;
-        T_[r][c] = a * xj;
-        v += a * xr[r];
-    }
-    V[g][c] = v;
-    __syncthreads();
-    double *Pp = P + (int64_t)blockIdx.x * 2 * CS_B;
-    if (threadIdx.x < CS_B) {
-        double u = 0.0;
-        for (int q = 0; q < CS_B; ++q) u += T_[threadIdx.x][q];
-        Pp[threadIdx.x] = u;
-    } else if (threadIdx.x < 2 * CS_B) {
-        const int q = (int)threadIdx.x - CS_B;
-        Pp[CS_B + q] = ((V[0][q] + V[1][q]) + V[2][q]) + V[3][q];
-    }
+constexpr int CS_RUN = 8;
+
+// index of the tile pair (I, J), I < J, among the strictly upper tiles
+__host__ __device__ inline int64_t cs_pair(int64_t I, int64_t J, int64_t nb) {
+    return I * (nb - 1) - I * (I - 1) / 2 + (J - I - 1);
 }
 
-// pass 2: one wave per block row I; pair index of (I, J), I <= J, is
-// I*nb - I*(I-1)/2 + (J - I)
-template <typename T>
-__global__ __launch_bounds__(64) void k_coarse_symv2(int64_t n, int nb, const double *__restrict__ P,
-                                                     const int64_t *__restrict__ freeidx, T *__restrict__ x,
-                                                     const int *done) {
+// the 16 values of one tile column for rows k = 0..15 of a row group
+__device__ __forceinline__ void cs_tile_load(brsrc_t r, uint32_t rowbytes, uint32_t bo, double (&a)[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = bld1(r, (uint32_t)k * rowbytes + bo, 0u, 0.0);
+}
+__device__ __forceinline__ double cs_tile_fma(const double (&a)[16], const double (&xi)[16], double xj,
+                                              double (&u)[16]) {
+    double vp = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        vp += a[k] * xi[k];
+        u[k] += a[k] * xj;
+    }
+    return vp;
+}
+
+__global__ __launch_bounds__(64) void k_coarse_symv_run(int64_t n, int nb, const double *__restrict__ A,
+                                                       const int4 *__restrict__ runs, const double *__restrict__ xg,
+                                                       double *__restrict__ V, double *__restrict__ U, const int *done) {
     if (done && *done) return;
-    const int I = (int)blockIdx.x, r = (int)threadIdx.x;
+    const int4 rn = runs[blockIdx.x];
+    const int I = rn.x, Ja = rn.y, nt = rn.z - rn.y;     // nt <= CS_RUN
+    const int c = (int)threadIdx.x;
+    const int64_t I0 = (int64_t)I * CS_B;
+    const int nr = (int)min<int64_t>(CS_B, n - I0);     // rows of this block row inside the matrix
+    const uint32_t rowbytes = (uint32_t)n * 8u;          // n * 8 * 16 < 2^32
+    __shared__ double vl[CS_RUN][CS_B];                 // column products of the run's tiles (lane c: column c)
+    for (int j = 0; j < nt; ++j) vl[j][c] = 0.0;
+    // A is read through one buffer per 16-row group that ends at the last row
+    // inside the matrix: rows past n read 0. Columns past n read finite
+    // entries of the next row (or 0); their products only reach rows past n,
+    // which pass 2 drops, or meet an x entry of 0.
+#pragma unroll 1
+    for (int r0 = 0; r0 < CS_B; r0 += 16) {
+        const int nrow = max(0, min(16, nr - r0));
+        // the group's base address and extent are wave-uniform (scalar registers)
+        const uint64_t ab = reinterpret_cast<uint64_t>(A + (I0 + r0) * n);
+        const uint64_t abu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ab >> 32)) << 32) |
+                             (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)ab);
+        const brsrc_t rA = buf_rsrc(reinterpret_cast<const void *>(abu),
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)nrow * rowbytes));
+        double xi[16], u[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double xr = xg[min<int64_t>(I0 + r0 + k, n - 1)];
+            xi[k] = k < nrow ? xr : 0.0;
+            u[k] = 0.0;
+        }
+        // two tiles per step: 32 row loads in flight per wave
+        int j = 0;
+#pragma unroll 1
+        for (; j + 1 < nt; j += 2) {
+            const int64_t c0 = (int64_t)(Ja + j) * CS_B + c, c1 = c0 + CS_B;
+            double a0[16], a1[16];
+            cs_tile_load(rA, rowbytes, (uint32_t)c0 * 8u, a0);
+            cs_tile_load(rA, rowbytes, (uint32_t)c1 * 8u, a1);
+            const double x0 = xg[min<int64_t>(c0, n - 1)] * (c0 < n ? 1.0 : 0.0);
+            const double x1 = xg[min<int64_t>(c1, n - 1)] * (c1 < n ? 1.0 : 0.0);
+            const double v0 = cs_tile_fma(a0, xi, x0, u);
+            const double v1 = cs_tile_fma(a1, xi, x1, u);
+            vl[j][c] += v0;
+            vl[j + 1][c] += v1;
+        }
+        if (j < nt) {
+            const int64_t c0 = (int64_t)(Ja + j) * CS_B + c;
+            double a0[16];
+            cs_tile_load(rA, rowbytes, (uint32_t)c0 * 8u, a0);
+            const double x0 = xg[min<int64_t>(c0, n - 1)] * (c0 < n ? 1.0 : 0.0);
+            vl[j][c] += cs_tile_fma(a0, xi, x0, u);
+        }
+        // transposing butterfly: at each step a lane keeps the half of its
+        // rows selected by its lane bit and adds its partner's copy of it
+#pragma unroll
+        for (int h = 8; h >= 1; h >>= 1) {
+            const bool hi = (c & h) != 0;
+#pragma unroll
+            for (int i = 0; i < h; ++i) {
+                const double keep = hi ? u[i + h] : u[i];
+                const double send = hi ? u[i] : u[i + h];
+                u[i] = keep + __shfl_xor(send, h, 64);
+            }
+        }
+        double s = u[0];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (c < 16) U[(int64_t)blockIdx.x * CS_B + r0 + c] = s;
+    }
+    for (int j = 0; j < nt; ++j)
+        if (Ja + j > I) V[cs_pair(I, Ja + j, nb) * CS_B + c] = vl[j][c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_coarse_symv_sum(int64_t n, int nb, const double *__restrict__ P,
+                                                         const int *__restrict__ rs,
+                                                         const int64_t *__restrict__ freeidx, T *__restrict__ x,
+                                                         const int *done) {
+    if (done && *done) return;
+    // P: the column products of the nb(nb-1)/2 strictly upper tiles, then the
+    // row products of the runs (row K's runs are rs[K] .. rs[K+1]-1)
+    const int K = (int)blockIdx.x, r = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int nv = K, nu = rs[K + 1] - rs[K];
+    const int nt = nv + nu, t0 = nt * w / 4, t1 = nt * (w + 1) / 4;
+    const int64_t ubase = ((int64_t)nb * (nb - 1) / 2 + rs[K] - nv) * CS_B;
     double y = 0.0;
-    for (int J = 0; J < I; ++J) {                      // v of pair (J, I)
-        const int64_t p = (int64_t)J * nb - (int64_t)J * (J - 1) / 2 + (I - J);
-        y += P[p * 2 * CS_B + CS_B + r];
+#pragma unroll 8
+    for (int t = t0; t < t1; ++t) y += P[(t < nv ? cs_pair(t, K, nb) * CS_B : ubase + (int64_t)t * CS_B) + r];
+    __shared__ double red[4][CS_B];
+    red[w][r] = y;
+    __syncthreads();
+    if (w == 0) {
+        const int64_t i = (int64_t)K * CS_B + r;
+        if (i < n) x[freeidx[i]] = (T)(((red[0][r] + red[1][r]) + red[2][r]) + red[3][r]);
     }
-    for (int J = I; J < nb; ++J) {                     // u of pair (I, J)
-        const int64_t p = (int64_t)I * nb - (int64_t)I * (I - 1) / 2 + (J - I);
-        y += P[p * 2 * CS_B + r];
-    }
-    const int64_t i = (int64_t)I * CS_B + r;
-    if (i < n) x[freeidx[i]] = (T)y;
 }
 
 __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,

@@ -82,12 +82,15 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
     add(8 * ne0); add(4 * ne0);                                // level-1 children's moduli (upper bound)
     add(8 * ne1 * 576); add(8 * (L > 2 ? ne2 * 576 : 1));
     add(8 * ncoarse * ncoarse); add(8 * ncoarse); add(8 * ncoarse); add(4);
-    // symmetric coarse solve (coarse systems of 16,384 DOFs and more): block
-    // pairs and per-pair partial products (upper bound: every coarse DOF free)
+    // symmetric coarse solve (coarse systems of 16,384 DOFs and more): the
+    // run table and the partial products (upper bound: every coarse DOF free,
+    // runs of one tile)
     if (ncoarse >= 16384) {
         const int64_t nb = (ncoarse + CS_B - 1) / CS_B;
-        add(sizeof(int2) * (size_t)(nb * (nb + 1) / 2));
-        add(8 * (size_t)(nb * (nb + 1) * CS_B));
+        const int64_t nruns = nb * (nb + 1) / 2;
+        add(sizeof(int4) * (size_t)nruns);
+        add(sizeof(int) * (size_t)(nb + 1));
+        add(8 * (size_t)((nb * (nb - 1) / 2 + nruns) * CS_B));
     }
     for (int k = 0; k < 6; ++k) add(8 * ndof0);
     // per-block partial sums: z chunks of at least one plane, of either the

@@ -78,12 +78,14 @@ struct tmg_solver {
     int64_t nc = 0;
     double *Ainv = nullptr, *cb = nullptr, *cx = nullptr;
     // large coarse systems: solve reading the upper block triangle of Ainv
-    // (k_coarse_symv1/2); below the threshold its second pass costs more
+    // (k_coarse_symv_run/sum); below the threshold its extra passes cost more
     // than the halved matrix read saves
     bool coarse_symv = false;
     int cs_nb = 0;                    // 64-row blocks
-    int2 *cs_pairs = nullptr;         // upper block pairs (I <= J)
-    double *cs_part = nullptr;        // per pair: A_IJ x_J and A_IJ' x_I
+    int cs_nruns = 0;                 // runs of consecutive upper tiles of one block row
+    int4 *cs_runs = nullptr;          // per run: block row, first and end tile column
+    int *cs_rs = nullptr;             // first run of each block row (cs_nb + 1)
+    double *cs_part = nullptr;        // column products per upper tile pair, then row products per run
     int *d_info = nullptr;
     rocblas_handle rb = nullptr;
     // PCG
@@ -551,11 +553,13 @@ template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
     if (h->coarse_symv) {
-        const int64_t npairs = (int64_t)h->cs_nb * (h->cs_nb + 1) / 2;
-        hipLaunchKernelGGL((k_coarse_symv1<T>), dim3((unsigned)npairs), dim3(TPB), 0, h->st, h->nc, h->Ainv,
-                           h->cs_pairs, lc.d_free, b, h->cs_part, done);
-        hipLaunchKernelGGL((k_coarse_symv2<T>), dim3((unsigned)h->cs_nb), dim3(CS_B), 0, h->st, h->nc, h->cs_nb,
-                           h->cs_part, lc.d_free, x, done);
+        const int64_t nb = h->cs_nb;
+        hipLaunchKernelGGL((k_gather<T>), dim3(nblk(h->nc)), dim3(TPB), 0, h->st, h->nc, lc.d_free, b, h->cb, done);
+        hipLaunchKernelGGL(k_coarse_symv_run, dim3((unsigned)h->cs_nruns), dim3(64), 0, h->st, h->nc, h->cs_nb,
+                           (const double *)h->Ainv, (const int4 *)h->cs_runs, (const double *)h->cb, h->cs_part,
+                           h->cs_part + nb * (nb - 1) / 2 * CS_B, done);
+        hipLaunchKernelGGL((k_coarse_symv_sum<T>), dim3((unsigned)h->cs_nb), dim3(TPB), 0, h->st, h->nc, h->cs_nb,
+                           (const double *)h->cs_part, (const int *)h->cs_rs, lc.d_free, x, done);
         return;
     }
     hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc, h->Ainv,
@@ -1073,7 +1077,7 @@ void destroy_all(tmg_solver *h) {
         dfree(h, lv.b); dfree(h, lv.x); dfree(h, lv.t); dfree(h, lv.d_free); dfree(h, lv.d_cidx);
     }
     dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d);
-    dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->cs_pairs); dfree(h, h->cs_part); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
+    dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->cs_runs); dfree(h, h->cs_rs); dfree(h, h->cs_part); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
     dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
@@ -1571,11 +1575,19 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->coarse_symv = h->nc >= 16384;
     if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
     h->cs_nb = (int)((h->nc + CS_B - 1) / CS_B);
-    std::vector<int2> cs_pairs;
+    std::vector<int4> cs_runs;
+    std::vector<int> cs_rs;
     if (h->coarse_symv) {
-        cs_pairs.reserve((size_t)h->cs_nb * (h->cs_nb + 1) / 2);
-        for (int I = 0; I < h->cs_nb; ++I)
-            for (int J = I; J < h->cs_nb; ++J) cs_pairs.push_back(make_int2(I, J));   // index I*nb - I(I-1)/2 + (J-I)
+        // runs of up to CS_RUN consecutive tiles (I, J >= I), row by row
+        // (k_coarse_symv_run keeps one run's column products in LDS)
+        int run = CS_RUN;
+        if (const char *ev = getenv("TMG_CS_RUN")) run = std::max(1, std::min(atoi(ev), CS_RUN));
+        for (int I = 0; I < h->cs_nb; ++I) {
+            cs_rs.push_back((int)cs_runs.size());
+            for (int J = I; J < h->cs_nb; J += run) cs_runs.push_back(make_int4(I, J, std::min(J + run, h->cs_nb), 0));
+        }
+        cs_rs.push_back((int)cs_runs.size());
+        h->cs_nruns = (int)cs_runs.size();
     }
     int rc = 0;
     const int64_t nem = l0.ne * h->nsub;
@@ -1587,8 +1599,9 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
-        (rc = dalloc(h, &h->cs_pairs, h->coarse_symv ? (int64_t)h->cs_nb * (h->cs_nb + 1) / 2 : 1)) ||
-        (rc = dalloc(h, &h->cs_part, h->coarse_symv ? (int64_t)h->cs_nb * (h->cs_nb + 1) * CS_B : 1)) ||
+        (rc = dalloc(h, &h->cs_runs, h->coarse_symv ? (int64_t)h->cs_nruns : 1)) ||
+        (rc = dalloc(h, &h->cs_rs, h->coarse_symv ? (int64_t)h->cs_nb + 1 : 1)) ||
+        (rc = dalloc(h, &h->cs_part, h->coarse_symv ? ((int64_t)h->cs_nb * (h->cs_nb - 1) / 2 + h->cs_nruns) * CS_B : 1)) ||
         (rc = dalloc(h, &h->px, l0.ndof)) || (rc = dalloc(h, &h->pr, l0.ndof)) || (rc = dalloc(h, &h->pd, l0.ndof)) ||
         (rc = dalloc(h, &h->pd2, l0.ndof)) ||
         (rc = dalloc(h, &h->pq, l0.ndof)) || (rc = dalloc(h, &h->pf, l0.ndof)))
@@ -1661,8 +1674,10 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         hipMemcpyAsync(h->kw32_d, h->kw32.g, sizeof(h->kw32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kwg64_d, h->kwg64.g, sizeof(h->kwg64.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
         hipMemcpyAsync(h->kwg32_d, h->kwg32.g, sizeof(h->kwg32.g), hipMemcpyHostToDevice, h->st) != hipSuccess ||
-        (!cs_pairs.empty() && hipMemcpyAsync(h->cs_pairs, cs_pairs.data(), sizeof(int2) * cs_pairs.size(),
-                                             hipMemcpyHostToDevice, h->st) != hipSuccess) ||
+        (!cs_runs.empty() && hipMemcpyAsync(h->cs_runs, cs_runs.data(), sizeof(int4) * cs_runs.size(),
+                                            hipMemcpyHostToDevice, h->st) != hipSuccess) ||
+        (!cs_rs.empty() && hipMemcpyAsync(h->cs_rs, cs_rs.data(), sizeof(int) * cs_rs.size(), hipMemcpyHostToDevice,
+                                          h->st) != hipSuccess) ||
         hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "copy of Ke failed";
         return fail(TMG_ERR_HIP);
