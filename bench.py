@@ -134,7 +134,7 @@ def iteration_bytes(prob, L, prec):
     late inside the direction update, the cycle's first Jacobi sweep inside
     the residual update); the fine level and level 1 matrix-free (moduli and
     free-DOF mask only); assembled symmetric stencils (126 coefficients per
-    node) below; the dense FP64 coarse inverse."""
+    node) below; the dense coarse inverse (its FP32 copy in the mixed cycle)."""
     s = 4 if prec == "mixed" else 8
     n0, ne0 = prob.ndof, prob.nelem
     b = 8 * (2 * n0 + ne0) + n0       # q = K d (read d, E, mask; write q)
@@ -150,7 +150,7 @@ def iteration_bytes(prob, L, prec):
         b += s * (nprev + n)          # restriction into level l
         b += s * (2 * nprev + n)      # prolongation back onto level l - 1
         if l == L - 1:
-            b += 8 * n * n + 2 * s * n     # dense coarse inverse times the coarse residual
+            b += s * n * n + 2 * s * n     # dense coarse inverse times the coarse residual
         elif l == 1:
             b += s * (3 * n + 8 * ex * ey * ez) * 2 + s * 3 * n   # two matrix-free sweeps + sweep from zero
         else:
@@ -444,7 +444,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (PCG, outer apply, coarse) + f32 (V-cycle)",
+        "dtype": "f64 (PCG, outer apply, coarse factorization) + f32 (V-cycle)",
         "data": "synthetic",
         "config": {
             "workload": f"cantilever {nelx}x{nely}x{nelz} Q1, {prob.nfree} free DOFs, {L}-level GMG V(1,1) "

@@ -1703,16 +1703,17 @@ __global__ __launch_bounds__(TPB) void k_scatter_full(int64_t ndof, const int64_
     out[d] = q >= 0 ? (T)v[q] : T(0);
 }
 
-// Coarsest-grid solve x = A^-1 b with the dense FP64 inverse: one block per
-// row, every thread's loads issued together, fixed-order block reduction.
+// Coarsest-grid solve x = A^-1 b with the dense inverse (FP64, or its FP32
+// copy for the FP32 cycle; products and sums in FP64): one block per row,
+// every thread's loads issued together, fixed-order block reduction.
 // Fixed DOFs of x are never written (they stay zero).
-template <typename T>
-__global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const double *__restrict__ A,
+template <typename T, typename TA = double>
+__global__ __launch_bounds__(TPB) void k_coarse_solve(int64_t n, const TA *__restrict__ A,
                                                       const int64_t *__restrict__ freeidx, const T *__restrict__ b,
                                                       T *__restrict__ x, const int *done) {
     if (done && *done) return;
     const int64_t row = blockIdx.x;
-    const double *Ar = A + row * n;
+    const TA *Ar = A + row * n;
     // four independent partial sums keep four index -> value -> product
     // chains in flight per thread
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -1767,22 +1768,28 @@ __host__ __device__ inline int64_t cs_pair(int64_t I, int64_t J, int64_t nb) {
 }
 
 // the 16 values of one tile column for rows k = 0..15 of a row group
-__device__ __forceinline__ void cs_tile_load(brsrc_t r, uint32_t rowbytes, uint32_t bo, double (&a)[16]) {
+template <typename TA>
+__device__ __forceinline__ void cs_tile_load(brsrc_t r, uint32_t rowbytes, uint32_t bo, TA (&a)[16]) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) a[k] = bld1(r, (uint32_t)k * rowbytes + bo, 0u, 0.0);
+    for (int k = 0; k < 16; ++k) a[k] = bld1(r, (uint32_t)k * rowbytes + bo, 0u, TA(0));
 }
-__device__ __forceinline__ double cs_tile_fma(const double (&a)[16], const double (&xi)[16], double xj,
-                                              double (&u)[16]) {
+template <typename TA>
+__device__ __forceinline__ double cs_tile_fma(const TA (&a)[16], const double (&xi)[16], double xj, double (&u)[16]) {
     double vp = 0.0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        vp += a[k] * xi[k];
-        u[k] += a[k] * xj;
+        const double ak = (double)a[k];
+        vp += ak * xi[k];
+        u[k] += ak * xj;
     }
     return vp;
 }
 
-__global__ __launch_bounds__(64) void k_coarse_symv_run(int64_t n, int nb, const double *__restrict__ A,
+// TA: the stored type of A (double, or the float copy the FP32 cycle reads;
+// products and sums are FP64 either way); TS tiles are loaded per step
+// (2 of double, 4 of float: 16 KB of row segments in flight per wave).
+template <typename TA, int TS>
+__global__ __launch_bounds__(64) void k_coarse_symv_run(int64_t n, int nb, const TA *__restrict__ A,
                                                        const int4 *__restrict__ runs, const double *__restrict__ xg,
                                                        double *__restrict__ V, double *__restrict__ U, const int *done) {
     if (done && *done) return;
@@ -1791,7 +1798,8 @@ __global__ __launch_bounds__(64) void k_coarse_symv_run(int64_t n, int nb, const
     const int c = (int)threadIdx.x;
     const int64_t I0 = (int64_t)I * CS_B;
     const int nr = (int)min<int64_t>(CS_B, n - I0);     // rows of this block row inside the matrix
-    const uint32_t rowbytes = (uint32_t)n * 8u;          // n * 8 * 16 < 2^32
+    constexpr uint32_t SA = sizeof(TA);
+    const uint32_t rowbytes = (uint32_t)n * SA;          // n * 8 * 16 < 2^32
     __shared__ double vl[CS_RUN][CS_B];                 // column products of the run's tiles (lane c: column c)
     for (int j = 0; j < nt; ++j) vl[j][c] = 0.0;
     // A is read through one buffer per 16-row group that ends at the last row
@@ -1814,27 +1822,27 @@ __global__ __launch_bounds__(64) void k_coarse_symv_run(int64_t n, int nb, const
             xi[k] = k < nrow ? xr : 0.0;
             u[k] = 0.0;
         }
-        // two tiles per step: 32 row loads in flight per wave
         int j = 0;
 #pragma unroll 1
-        for (; j + 1 < nt; j += 2) {
-            const int64_t c0 = (int64_t)(Ja + j) * CS_B + c, c1 = c0 + CS_B;
-            double a0[16], a1[16];
-            cs_tile_load(rA, rowbytes, (uint32_t)c0 * 8u, a0);
-            cs_tile_load(rA, rowbytes, (uint32_t)c1 * 8u, a1);
-            const double x0 = xg[min<int64_t>(c0, n - 1)] * (c0 < n ? 1.0 : 0.0);
-            const double x1 = xg[min<int64_t>(c1, n - 1)] * (c1 < n ? 1.0 : 0.0);
-            const double v0 = cs_tile_fma(a0, xi, x0, u);
-            const double v1 = cs_tile_fma(a1, xi, x1, u);
-            vl[j][c] += v0;
-            vl[j + 1][c] += v1;
+        for (; j + TS <= nt; j += TS) {
+            TA a[TS][16];
+            double xjv[TS];
+#pragma unroll
+            for (int t = 0; t < TS; ++t) {
+                const int64_t ct = (int64_t)(Ja + j + t) * CS_B + c;
+                cs_tile_load<TA>(rA, rowbytes, (uint32_t)ct * SA, a[t]);
+                xjv[t] = xg[min<int64_t>(ct, n - 1)] * (ct < n ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int t = 0; t < TS; ++t) vl[j + t][c] += cs_tile_fma<TA>(a[t], xi, xjv[t], u);
         }
-        if (j < nt) {
-            const int64_t c0 = (int64_t)(Ja + j) * CS_B + c;
-            double a0[16];
-            cs_tile_load(rA, rowbytes, (uint32_t)c0 * 8u, a0);
-            const double x0 = xg[min<int64_t>(c0, n - 1)] * (c0 < n ? 1.0 : 0.0);
-            vl[j][c] += cs_tile_fma(a0, xi, x0, u);
+#pragma unroll 1
+        for (; j < nt; ++j) {
+            const int64_t ct = (int64_t)(Ja + j) * CS_B + c;
+            TA a0[16];
+            cs_tile_load<TA>(rA, rowbytes, (uint32_t)ct * SA, a0);
+            const double x0 = xg[min<int64_t>(ct, n - 1)] * (ct < n ? 1.0 : 0.0);
+            vl[j][c] += cs_tile_fma<TA>(a0, xi, x0, u);
         }
         // transposing butterfly: at each step a lane keeps the half of its
         // rows selected by its lane bit and adds its partner's copy of it

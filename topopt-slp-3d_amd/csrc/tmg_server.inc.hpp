@@ -82,6 +82,7 @@ size_t estimate_bytes(int nelx, int nely, int nelz, int L, int prec, int64_t nfr
     add(8 * ne0); add(4 * ne0);                                // level-1 children's moduli (upper bound)
     add(8 * ne1 * 576); add(8 * (L > 2 ? ne2 * 576 : 1));
     add(8 * ncoarse * ncoarse); add(8 * ncoarse); add(8 * ncoarse); add(4);
+    if (prec == TMG_MIXED) add(4 * ncoarse * ncoarse);         // FP32 copy of the coarse inverse
     // symmetric coarse solve (coarse systems of 16,384 DOFs and more): the
     // run table and the partial products (upper bound: every coarse DOF free,
     // runs of one tile)

@@ -77,6 +77,10 @@ struct tmg_solver {
     // coarsest grid
     int64_t nc = 0;
     double *Ainv = nullptr, *cb = nullptr, *cx = nullptr;
+    // mixed precision: the FP32 cycle's coarse solve reads an FP32 copy of
+    // the inverse (half the bytes; the FP64 hierarchy reads Ainv)
+    bool coarse_f32 = false;
+    float *Ainv32 = nullptr;
     // large coarse systems: solve reading the upper block triangle of Ainv
     // (k_coarse_symv_run/sum); below the threshold its extra passes cost more
     // than the halved matrix read saves
@@ -552,18 +556,28 @@ void apply_pcg_q(tmg_solver *h, bool consumer_masks, const int *done) {
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
+    const bool f32 = std::is_same<T, float>::value && h->coarse_f32;
     if (h->coarse_symv) {
         const int64_t nb = h->cs_nb;
         hipLaunchKernelGGL((k_gather<T>), dim3(nblk(h->nc)), dim3(TPB), 0, h->st, h->nc, lc.d_free, b, h->cb, done);
-        hipLaunchKernelGGL(k_coarse_symv_run, dim3((unsigned)h->cs_nruns), dim3(64), 0, h->st, h->nc, h->cs_nb,
-                           (const double *)h->Ainv, (const int4 *)h->cs_runs, (const double *)h->cb, h->cs_part,
-                           h->cs_part + nb * (nb - 1) / 2 * CS_B, done);
+        if (f32)
+            hipLaunchKernelGGL((k_coarse_symv_run<float, 4>), dim3((unsigned)h->cs_nruns), dim3(64), 0, h->st, h->nc,
+                               h->cs_nb, (const float *)h->Ainv32, (const int4 *)h->cs_runs, (const double *)h->cb,
+                               h->cs_part, h->cs_part + nb * (nb - 1) / 2 * CS_B, done);
+        else
+            hipLaunchKernelGGL((k_coarse_symv_run<double, 2>), dim3((unsigned)h->cs_nruns), dim3(64), 0, h->st, h->nc,
+                               h->cs_nb, (const double *)h->Ainv, (const int4 *)h->cs_runs, (const double *)h->cb,
+                               h->cs_part, h->cs_part + nb * (nb - 1) / 2 * CS_B, done);
         hipLaunchKernelGGL((k_coarse_symv_sum<T>), dim3((unsigned)h->cs_nb), dim3(TPB), 0, h->st, h->nc, h->cs_nb,
                            (const double *)h->cs_part, (const int *)h->cs_rs, lc.d_free, x, done);
         return;
     }
-    hipLaunchKernelGGL((k_coarse_solve<T>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc, h->Ainv,
-                       lc.d_free, b, x, done);
+    if (f32)
+        hipLaunchKernelGGL((k_coarse_solve<T, float>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc,
+                           (const float *)h->Ainv32, lc.d_free, b, x, done);
+    else
+        hipLaunchKernelGGL((k_coarse_solve<T, double>), dim3((unsigned)h->nc), dim3(TPB), 0, h->st, h->nc,
+                           (const double *)h->Ainv, lc.d_free, b, x, done);
 }
 
 // One V(nu1,nu2) cycle from a zero initial guess at level l
@@ -1079,7 +1093,7 @@ void destroy_all(tmg_solver *h) {
     dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d);
     dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->cs_runs); dfree(h, h->cs_rs); dfree(h, h->cs_part); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
-    dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
+    dfree(h, h->Ainv32); dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
     dfree(h, h->partial); dfree(h, h->partial2); dfree(h, h->d_sc);
     if (h->h_sc && h->own_hsc) (void)hipHostFree(h->h_sc);
@@ -1214,6 +1228,9 @@ int refresh(tmg_solver *h) {
             return TMG_ERR_HIP;
         }
         hipLaunchKernelGGL(k_symmetrize_upper, dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc, h->Ainv);
+        if (h->coarse_f32)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc * nc,
+                               (const double *)h->Ainv, h->Ainv32, (const int *)nullptr);
     }
     TRY(check_launch(h));
     HIPX(hipStreamSynchronize(h->st));
@@ -1573,6 +1590,8 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     // the LDS-staged apply addresses its vectors through 32-bit buffer offsets
     if ((uint64_t)h->lev[0].ndof * 8 >= ((uint64_t)1 << 32) - 4096) h->ebe_lds = false;
     h->coarse_symv = h->nc >= 16384;
+    h->coarse_f32 = precision == TMG_MIXED;
+    if (const char *ev = getenv("TMG_COARSE_F32")) h->coarse_f32 = precision == TMG_MIXED && ev[0] != '0';
     if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
     h->cs_nb = (int)((h->nc + CS_B - 1) / CS_B);
     std::vector<int4> cs_runs;
@@ -1598,6 +1617,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
         (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
+        (rc = dalloc(h, &h->Ainv32, h->coarse_f32 ? h->nc * h->nc : 1)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
         (rc = dalloc(h, &h->cs_runs, h->coarse_symv ? (int64_t)h->cs_nruns : 1)) ||
         (rc = dalloc(h, &h->cs_rs, h->coarse_symv ? (int64_t)h->cs_nb + 1 : 1)) ||
