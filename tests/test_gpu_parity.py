@@ -477,6 +477,45 @@ def test_symmetric_coarse_solve_matches_full_and_oracle(oracle_mod, monkeypatch,
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nelx,nely,nelz,L,seed,leaf", [(16, 8, 8, 3, 31, 4), (24, 12, 12, 3, 32, 8),
+                                                        (32, 16, 16, 3, 33, 16), (64, 32, 32, 3, 34, 64)])
+def test_nested_dissection_coarse_solve_matches_dense_and_oracle(oracle_mod, monkeypatch, precision, nelx, nely,
+                                                                 nelz, L, seed, leaf):
+    """The coarse solve by nested dissection and multifrontal Cholesky (the
+    default from 16,384 coarse DOFs) forced on small coarse grids, with leaves
+    of 4 ... 64 nodes so that the dissection tree has 2 to 6 levels: the
+    V-cycle equals the dense-inverse solve's to rounding, and the PCG solve
+    meets the oracle."""
+    _say(f"nested-dissection coarse solve {precision} {nelx}x{nely}x{nelz} L={L} leaf={leaf}")
+    prob = P.cantilever(nelx, nely, nelz)
+    x = P.random_density(nelx, nely, nelz, seed)
+    o = _oracle(oracle_mod, prob, x, L)
+    monkeypatch.setenv("TMG_COARSE_ND", "0")
+    monkeypatch.setenv("TMG_COARSE_F32", "0")
+    s0 = _setup(prob, x, L, precision)
+    monkeypatch.setenv("TMG_COARSE_ND", "1")
+    monkeypatch.setenv("TMG_ND_LEAF", str(leaf))
+    s1 = _setup(prob, x, L, precision)
+    b = np.random.default_rng(seed).standard_normal(prob.nfree)
+    v0, v1 = s0.vcycle(b), s1.vcycle(b)
+    assert _rel(v1, v0) < (1e-12 if precision == "fp64" else 1e-5)
+    if precision == "fp64":
+        assert _rel(v1, o.vcycle(b)) < 1e-8
+    u_or, it_or, _ = o.solve(prob.f)
+    u, it, rr, rc = s1.solve(prob.f)
+    assert rc == 0 and rr <= 1e-8
+    if precision == "fp64":
+        assert abs(it - it_or) <= 1
+    else:
+        # the FP32 cycle's iteration count moves with rounding; the dense
+        # solve's count on the same problem is the reference for it
+        _, it0, _, _ = s0.solve(prob.f)
+        _say(f"  mixed: {it} iterations, dense coarse solve {it0}, oracle (FP64) {it_or}")
+        assert abs(it - it0) <= 2 and it <= 1.5 * it_or + 3
+    _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("nelx,nely,nelz,L,seed", [(16, 8, 8, 3, 23), (16, 8, 8, 4, 24)])
 def test_fmv_cycle_matches_oracle(oracle_mod, precision, nelx, nely, nelz, L, seed):
     """mg.cycle = 2: the FMV cycle of Implementation/FMVCycle.m as the

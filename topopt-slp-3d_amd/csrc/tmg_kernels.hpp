@@ -1889,6 +1889,150 @@ __global__ __launch_bounds__(TPB) void k_coarse_symv_sum(int64_t n, int nb, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// Coarsest grid by nested dissection (large coarse systems): the free coarse
+// DOFs are ordered by a geometric nested dissection of the node grid (node
+// planes as separators); each tree node is a front with p pivots (its own
+// DOFs) and b boundary DOFs (the free DOFs of the ancestor separators next to
+// its region). The multifrontal Cholesky factorization leaves in the first p
+// columns of each front's column-major (p+b) x (p+b) block
+//   M = [L_PP^-1 ; L_BP L_PP^-1],
+// so that both triangular solves become matrix-vector products:
+//   forward  (leaves first):  [y_P ; t] = M (b_P - children's updates on P)
+//   backward (root first):    x_P = M' [y_P ; -x_B]
+// where t (plus the children's updates on B) is the front's update vector
+// that its parent gathers. The forward product reads a row-major copy of M
+// (Mt), the backward one M itself, so that both read contiguous rows or
+// columns one wave at a time. Every sum has a fixed order.
+// ---------------------------------------------------------------------------
+struct NdFront {
+    int p, b, ld;            // pivots, boundary DOFs, leading dimension p + b
+    int child[2];            // children fronts or -1
+    int64_t P_off, B_off;    // compact coarse DOF indices of the pivots / boundary (nd_P, nd_B)
+    int64_t F_off;           // column-major ld x ld block in nd_F
+    int64_t T_off;           // update vector (b entries) in nd_T
+    int64_t Mt_off;          // row-major copy of M (ld rows of p entries) in nd_Mt
+    int64_t cmap_off[2];     // per child: ld entries, the position in the child's boundary or -1
+};
+
+// original operator entries of every front (lower triangle; F zeroed first)
+__global__ __launch_bounds__(TPB) void k_nd_assemble(int64_t nent, const int64_t *__restrict__ src,
+                                                     const int64_t *__restrict__ dst, const double *__restrict__ S,
+                                                     double *__restrict__ F) {
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e < nent) F[dst[e]] = S[src[e]];
+}
+
+// extend-add of the children's update matrices (lower triangles) into the
+// fronts of one height: one block per (front, column l), rows k >= l
+__global__ __launch_bounds__(TPB) void k_nd_extend_add(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
+                                                       const int *__restrict__ cmap, double *__restrict__ F) {
+    const int2 w = work[blockIdx.x];
+    const NdFront f = fr[w.x];
+    const int l = w.y;
+    double *Fs = F + f.F_off + (int64_t)l * f.ld;
+    for (int c = 0; c < 2; ++c) {
+        if (f.child[c] < 0) continue;
+        const NdFront fc = fr[f.child[c]];
+        const int *cm = cmap + f.cmap_off[c];
+        const int jl = cm[l];
+        if (jl < 0) continue;
+        const double *Uc = F + fc.F_off + fc.p + (int64_t)(fc.p + jl) * fc.ld;   // column jl of the child's update
+        for (int k = l + (int)threadIdx.x; k < f.ld; k += TPB) {
+            const int ik = cm[k];
+            if (ik >= 0) Fs[k] += Uc[ik];
+        }
+    }
+}
+
+// the children's update vectors at position k of front f, in child order
+__device__ __forceinline__ double nd_child_sum(const NdFront &f, const NdFront *__restrict__ fr,
+                                               const int *__restrict__ cmap, const double *__restrict__ Tv, int k) {
+    double a = 0.0;
+    for (int c = 0; c < 2; ++c) {
+        if (f.child[c] < 0) continue;
+        const int ik = cmap[f.cmap_off[c] + k];
+        if (ik >= 0) a += Tv[fr[f.child[c]].T_off + ik];
+    }
+    return a;
+}
+
+// row-major copy of M: one block per (front, pivot column j), Mt[j + k p] = M[k][j]
+__global__ __launch_bounds__(TPB) void k_nd_transpose(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
+                                                      const double *__restrict__ F, double *__restrict__ Mt) {
+    const int2 w = work[blockIdx.x];
+    const NdFront f = fr[w.x];
+    const int j = w.y;
+    const double *Mj = F + f.F_off + (int64_t)j * f.ld;
+    double *out = Mt + f.Mt_off + j;
+    for (int k = (int)threadIdx.x; k < f.ld; k += TPB) out[(int64_t)k * f.p] = Mj[k];
+}
+
+// forward solve of the fronts of one height: RPW consecutive rows k of M per
+// wave (4 waves per block), the lanes over the row's contiguous pivot columns
+// (j <= k on the lower-triangular pivot rows). z (the front's pivot right-hand side) is
+// formed in LDS by every block of the front.
+template <typename T, int RPW>
+__global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
+                                                const int64_t *__restrict__ Pidx, const int *__restrict__ cmap,
+                                                const double *__restrict__ Mt, const int64_t *__restrict__ freeidx,
+                                                const T *__restrict__ bl, double *__restrict__ y,
+                                                double *__restrict__ Tv, const int *done) {
+    if (done && *done) return;
+    extern __shared__ double nd_sh[];
+    const int2 w = work[blockIdx.x];
+    const NdFront f = fr[w.x];
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const int64_t *P = Pidx + f.P_off;
+    for (int j = (int)threadIdx.x; j < f.p; j += TPB)
+        nd_sh[j] = (double)bl[freeidx[P[j]]] - nd_child_sum(f, fr, cmap, Tv, j);
+    __syncthreads();
+    for (int r = 0; r < RPW; ++r) {
+        const int k = w.y + wv * RPW + r;      // RPW consecutive rows per wave
+        if (k >= f.ld) return;
+        const int jn = k < f.p ? k + 1 : f.p;
+        const double *Mk = Mt + f.Mt_off + (int64_t)k * f.p;
+        double s = 0.0;
+        for (int j = lane; j < jn; j += 64) s += Mk[j] * nd_sh[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) {
+            if (k < f.p) y[P[k]] = s;
+            else Tv[f.T_off + (k - f.p)] = s + nd_child_sum(f, fr, cmap, Tv, k);
+        }
+    }
+}
+
+// backward solve of the fronts of one height: one wave per pivot column j
+// (4 per block): x_P[j] = sum_{k >= j} M[k][j] q[k], q = [y_P ; -x_B]
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_nd_bwd(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
+                                                const int64_t *__restrict__ Pidx, const int64_t *__restrict__ Bidx,
+                                                const double *__restrict__ F, const int64_t *__restrict__ freeidx,
+                                                const double *__restrict__ y, double *__restrict__ xc,
+                                                T *__restrict__ xl, const int *done) {
+    if (done && *done) return;
+    extern __shared__ double nd_sh[];
+    const int2 w = work[blockIdx.x];
+    const NdFront f = fr[w.x];
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const int64_t *P = Pidx + f.P_off;
+    const int64_t *B = Bidx + f.B_off;
+    for (int k = (int)threadIdx.x; k < f.ld; k += TPB) nd_sh[k] = k < f.p ? y[P[k]] : -xc[B[k - f.p]];
+    __syncthreads();
+    const int j = w.y + wv;
+    if (j >= f.p) return;
+    const double *Mj = F + f.F_off + (int64_t)j * f.ld;
+    double s = 0.0;
+    for (int k = j + lane; k < f.ld; k += 64) s += Mj[k] * nd_sh[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) {
+        xc[P[j]] = s;
+        xl[freeidx[P[j]]] = (T)s;
+    }
+}
+
 __global__ __launch_bounds__(TPB) void k_gemv(int64_t n, const double *__restrict__ A, const double *__restrict__ x,
                                               double *__restrict__ y, const int *done) {
     if (done && *done) return;

@@ -91,6 +91,20 @@ struct tmg_solver {
     int *cs_rs = nullptr;             // first run of each block row (cs_nb + 1)
     double *cs_part = nullptr;        // column products per upper tile pair, then row products per run
     int *d_info = nullptr;
+    // large coarse systems: multifrontal Cholesky over a nested dissection of
+    // the coarse node grid instead of the dense inverse (tmg_coarse_nd.inc.hpp)
+    bool coarse_nd = false;
+    int nd_nfront = 0, nd_maxld = 0, nd_nheight = 0;
+    int nd_rpb = 4;                   // forward-solve rows per block (4 waves x 1 row, or 4 x 4)
+    std::vector<NdFront> nd_fr;       // host copy of the front table (height order)
+    std::vector<int> nd_hstart;       // first front of each height (nd_nheight + 1)
+    std::vector<int> nd_ea_start, nd_fw_start, nd_bw_start;   // work-list ranges per height
+    NdFront *nd_fronts = nullptr;
+    int64_t *nd_P = nullptr, *nd_B = nullptr, *nd_src = nullptr, *nd_dst = nullptr;
+    int *nd_cmap = nullptr, *nd_info = nullptr;
+    int2 *nd_ea = nullptr, *nd_fw = nullptr, *nd_bw = nullptr, *nd_tp = nullptr;
+    double *nd_F = nullptr, *nd_Mt = nullptr, *nd_T = nullptr, *nd_y = nullptr, *nd_xc = nullptr;
+    int64_t nd_nent = 0, nd_Fsize = 0, nd_ntp = 0;
     rocblas_handle rb = nullptr;
     // PCG
     double *px = nullptr, *pr = nullptr, *pd = nullptr, *pd2 = nullptr, *pq = nullptr, *pf = nullptr;
@@ -266,6 +280,8 @@ void dfree(tmg_solver *h, T *&p) {
     }
     p = nullptr;
 }
+
+#include "tmg_coarse_nd.inc.hpp"
 
 // ---------------------------------------------------------------------------
 // operator applies
@@ -556,6 +572,10 @@ void apply_pcg_q(tmg_solver *h, bool consumer_masks, const int *done) {
 template <typename T>
 void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     DevLevel &lc = h->lev[h->L - 1];
+    if (h->coarse_nd) {
+        nd_solve<T>(h, b, x, done);
+        return;
+    }
     const bool f32 = std::is_same<T, float>::value && h->coarse_f32;
     if (h->coarse_symv) {
         const int64_t nb = h->cs_nb;
@@ -1093,6 +1113,7 @@ void destroy_all(tmg_solver *h) {
     dfree(h, h->K0_64); dfree(h, h->K0_32); dfree(h, h->kw64_d); dfree(h, h->kw32_d);
     dfree(h, h->kwg64_d); dfree(h, h->kwg32_d); dfree(h, h->cs_runs); dfree(h, h->cs_rs); dfree(h, h->cs_part); dfree(h, h->E1_64); dfree(h, h->E1_32); dfree(h, h->E64); dfree(h, h->E32); dfree(h, h->xdens);
     dfree(h, h->Ahat[0]); dfree(h, h->Ahat[1]);
+    nd_free(h);
     dfree(h, h->Ainv32); dfree(h, h->Ainv); dfree(h, h->cb); dfree(h, h->cx); dfree(h, h->d_info);
     dfree(h, h->px); dfree(h, h->pr); dfree(h, h->pd); dfree(h, h->pd2); dfree(h, h->pq); dfree(h, h->pf);
     dfree(h, h->partial); dfree(h, h->partial2); dfree(h, h->d_sc);
@@ -1201,9 +1222,16 @@ int refresh(tmg_solver *h) {
         hipLaunchKernelGGL(k_diag_stencil, dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, lv.nn, lv.S64, lv.mask, om, lv.inv64,
                            h->prec == TMG_MIXED ? lv.inv32 : nullptr);
     }
-    // coarsest: dense operator, Cholesky, inverse (Implementation/SetupGMG.m:51-60)
+    // coarsest (Implementation/SetupGMG.m:51-60): multifrontal Cholesky over
+    // a nested dissection, or the dense operator, Cholesky and inverse
     DevLevel &lc = h->lev[h->L - 1];
     const int64_t nc = h->nc;
+    if (h->coarse_nd) {
+        TRY(nd_factor(h));
+        HIPX(hipStreamSynchronize(h->st));
+        h->have_op = true;
+        return 0;
+    }
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(nc * nc)), dim3(TPB), 0, h->st, nc * nc, 0.0, h->Ainv);
     hipLaunchKernelGGL(k_dense_from_stencil, dim3(nblk(lc.nn * 14)), dim3(TPB), 0, h->st, lc.nx, lc.ny, lc.nz, lc.S64,
                        lc.d_cidx, nc, h->Ainv);
@@ -1593,6 +1621,19 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     h->coarse_f32 = precision == TMG_MIXED;
     if (const char *ev = getenv("TMG_COARSE_F32")) h->coarse_f32 = precision == TMG_MIXED && ev[0] != '0';
     if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
+    // large coarse systems: nested dissection and a multifrontal factorization
+    // (falls back to the dense inverse when its fronts do not fit the solve)
+    h->coarse_nd = h->nc >= 16384;
+    if (const char *ev = getenv("TMG_COARSE_ND")) h->coarse_nd = ev[0] != '0';
+    NdHost ndh;
+    if (h->coarse_nd) {
+        int leaf = 64;
+        if (const char *ev = getenv("TMG_ND_LEAF")) leaf = std::max(1, atoi(ev));
+        if (const char *ev = getenv("TMG_ND_RPB")) h->nd_rpb = atoi(ev) == 16 ? 16 : 4;
+        if (!nd_symbolic(h->lev[h->L - 1], leaf, h->nd_rpb, ndh)) h->coarse_nd = false;
+    }
+    if (h->coarse_nd) h->coarse_symv = h->coarse_f32 = false;
+    const int64_t ncd = h->coarse_nd ? 1 : h->nc;   // dense inverse size (per side)
     h->cs_nb = (int)((h->nc + CS_B - 1) / CS_B);
     std::vector<int4> cs_runs;
     std::vector<int> cs_rs;
@@ -1616,7 +1657,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->E1_64, h->l1_wht ? l0.ne : 1)) || (rc = dalloc(h, &h->E1_32, h->l1_wht ? l0.ne : 1)) ||
         (rc = dalloc(h, &h->E64, nem)) || (rc = dalloc(h, &h->E32, nem)) || (rc = dalloc(h, &h->xdens, nem)) ||
         (rc = dalloc(h, &h->Ahat[0], h->lev[1].ne * 576)) ||
-        (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, h->nc * h->nc)) ||
+        (rc = dalloc(h, &h->Ahat[1], h->L > 2 ? h->lev[2].ne * 576 : 1)) || (rc = dalloc(h, &h->Ainv, ncd * ncd)) ||
         (rc = dalloc(h, &h->Ainv32, h->coarse_f32 ? h->nc * h->nc : 1)) ||
         (rc = dalloc(h, &h->cb, h->nc)) || (rc = dalloc(h, &h->cx, h->nc)) || (rc = dalloc(h, &h->d_info, 1)) ||
         (rc = dalloc(h, &h->cs_runs, h->coarse_symv ? (int64_t)h->cs_nruns : 1)) ||
@@ -1626,6 +1667,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         (rc = dalloc(h, &h->pd2, l0.ndof)) ||
         (rc = dalloc(h, &h->pq, l0.ndof)) || (rc = dalloc(h, &h->pf, l0.ndof)))
         return fail(rc);
+    if (h->coarse_nd && (rc = nd_upload(h, ndh)) != 0) return fail(rc);
     // z-chunk of the fine apply: enough blocks to fill the chip
     {
         const int64_t pb = nblk((int64_t)l0.nx * l0.ny);
