@@ -22,7 +22,7 @@ int check_launch(tmg_solver *h);
 
 struct NdHost {
     std::vector<NdFront> fr;
-    std::vector<int64_t> P, B, src, dst;
+    std::vector<int64_t> P, B, src, dst, bsrc, tsrc;
     std::vector<int> cmap, hstart, ea_start, fw_start, bw_start;
     std::vector<int2> ea, fw, bw, tp;
     int64_t Fsize = 0, Tsize = 0, Mtsize = 0;
@@ -154,6 +154,9 @@ bool nd_symbolic(const DevLevel &lc, int leaf, int rpb, NdHost &o) {
     o.Mtsize = moff;
     // child maps and operator entries
     o.cmap.assign(coff, -1);
+    o.tsrc.assign(coff, -1);
+    o.bsrc.resize(o.P.size());
+    for (size_t k = 0; k < o.P.size(); ++k) o.bsrc[k] = lc.free_full[o.P[k]];
     std::vector<int> pos(nc, -1), cpos(nc, -1);
     for (int f = 0; f < nf; ++f) {
         const NdFront &F = o.fr[f];
@@ -168,7 +171,11 @@ bool nd_symbolic(const DevLevel &lc, int leaf, int rpb, NdHost &o) {
                 if (pos[CB[k]] < 0) return false;    // a child's boundary lies in its parent's front
                 cpos[CB[k]] = k;
             }
-            for (int k = 0; k < F.ld; ++k) o.cmap[F.cmap_off[c] + k] = cpos[dof(k)];
+            for (int k = 0; k < F.ld; ++k) {
+                const int ck = cpos[dof(k)];
+                o.cmap[F.cmap_off[c] + k] = ck;
+                o.tsrc[F.cmap_off[c] + k] = ck >= 0 ? C.T_off + ck : -1;
+            }
             for (int k = 0; k < C.b; ++k) cpos[CB[k]] = -1;
         }
         // lower-triangle entries A(i, j), j a pivot, i in the front
@@ -241,6 +248,8 @@ int nd_upload(tmg_solver *h, NdHost &o) {
     TRY(dalloc(h, &h->nd_src, h->nd_nent));
     TRY(dalloc(h, &h->nd_dst, h->nd_nent));
     TRY(dalloc(h, &h->nd_cmap, (int64_t)o.cmap.size()));
+    TRY(dalloc(h, &h->nd_tsrc, (int64_t)o.tsrc.size()));
+    TRY(dalloc(h, &h->nd_bsrc, (int64_t)o.bsrc.size()));
     TRY(dalloc(h, &h->nd_info, (int64_t)o.fr.size()));
     TRY(dalloc(h, &h->nd_ea, (int64_t)o.ea.size()));
     TRY(dalloc(h, &h->nd_fw, (int64_t)o.fw.size()));
@@ -259,6 +268,8 @@ int nd_upload(tmg_solver *h, NdHost &o) {
         !up(h->nd_src, o.src.data(), sizeof(int64_t) * o.src.size()) ||
         !up(h->nd_dst, o.dst.data(), sizeof(int64_t) * o.dst.size()) ||
         !up(h->nd_cmap, o.cmap.data(), sizeof(int) * o.cmap.size()) ||
+        !up(h->nd_tsrc, o.tsrc.data(), sizeof(int64_t) * o.tsrc.size()) ||
+        !up(h->nd_bsrc, o.bsrc.data(), sizeof(int64_t) * o.bsrc.size()) ||
         !up(h->nd_ea, o.ea.data(), sizeof(int2) * o.ea.size()) || !up(h->nd_fw, o.fw.data(), sizeof(int2) * o.fw.size()) ||
         !up(h->nd_bw, o.bw.data(), sizeof(int2) * o.bw.size()) || !up(h->nd_tp, o.tp.data(), sizeof(int2) * o.tp.size()) ||
         hipStreamSynchronize(h->st) != hipSuccess) {
@@ -271,7 +282,8 @@ int nd_upload(tmg_solver *h, NdHost &o) {
 void nd_free(tmg_solver *h) {
     dfree(h, h->nd_xc); dfree(h, h->nd_y); dfree(h, h->nd_T); dfree(h, h->nd_tp); dfree(h, h->nd_Mt); dfree(h, h->nd_F);
     dfree(h, h->nd_bw);
-    dfree(h, h->nd_fw); dfree(h, h->nd_ea); dfree(h, h->nd_info); dfree(h, h->nd_cmap); dfree(h, h->nd_dst);
+    dfree(h, h->nd_fw); dfree(h, h->nd_ea); dfree(h, h->nd_info); dfree(h, h->nd_bsrc); dfree(h, h->nd_tsrc);
+    dfree(h, h->nd_cmap); dfree(h, h->nd_dst);
     dfree(h, h->nd_src); dfree(h, h->nd_B); dfree(h, h->nd_P); dfree(h, h->nd_fronts);
 }
 
@@ -348,20 +360,19 @@ int nd_factor(tmg_solver *h) {
 // x = A^-1 b on the coarsest level (b, x: level vectors of the cycle's type)
 template <typename T>
 void nd_solve(tmg_solver *h, const T *b, T *x, const int *done) {
-    const DevLevel &lc = h->lev[h->L - 1];
     const size_t sh = sizeof(double) * (size_t)h->nd_maxld;
     for (int hh = 0; hh < h->nd_nheight; ++hh) {
         const int w0 = h->nd_fw_start[hh], w1 = h->nd_fw_start[hh + 1];
         auto kf = h->nd_rpb == 16 ? k_nd_fwd<T, 4> : k_nd_fwd<T, 1>;
         hipLaunchKernelGGL(kf, dim3((unsigned)(w1 - w0)), dim3(TPB), sh, h->st, (const int2 *)h->nd_fw + w0,
-                           (const NdFront *)h->nd_fronts, (const int64_t *)h->nd_P, (const int *)h->nd_cmap,
-                           (const double *)h->nd_Mt, (const int64_t *)lc.d_free, b, h->nd_y, h->nd_T, done);
+                           (const NdFront *)h->nd_fronts, (const int64_t *)h->nd_P, (const int64_t *)h->nd_bsrc,
+                           (const int64_t *)h->nd_tsrc, (const double *)h->nd_Mt, b, h->nd_y, h->nd_T, done);
     }
     for (int hh = h->nd_nheight - 1; hh >= 0; --hh) {
         const int w0 = h->nd_bw_start[hh], w1 = h->nd_bw_start[hh + 1];
         hipLaunchKernelGGL((k_nd_bwd<T>), dim3((unsigned)(w1 - w0)), dim3(TPB), sh, h->st, (const int2 *)h->nd_bw + w0,
                            (const NdFront *)h->nd_fronts, (const int64_t *)h->nd_P, (const int64_t *)h->nd_B,
-                           (const double *)h->nd_F, (const int64_t *)lc.d_free, (const double *)h->nd_y, h->nd_xc, x,
+                           (const double *)h->nd_F, (const int64_t *)h->nd_bsrc, (const double *)h->nd_y, h->nd_xc, x,
                            done);
     }
 }

@@ -1946,13 +1946,15 @@ __global__ __launch_bounds__(TPB) void k_nd_extend_add(const int2 *__restrict__ 
 }
 
 // the children's update vectors at position k of front f, in child order
-__device__ __forceinline__ double nd_child_sum(const NdFront &f, const NdFront *__restrict__ fr,
-                                               const int *__restrict__ cmap, const double *__restrict__ Tv, int k) {
+// (tsrc: per child and front position, the entry of the child's update vector
+// in Tv, or -1)
+__device__ __forceinline__ double nd_child_sum(const NdFront &f, const int64_t *__restrict__ tsrc,
+                                               const double *__restrict__ Tv, int k) {
     double a = 0.0;
     for (int c = 0; c < 2; ++c) {
         if (f.child[c] < 0) continue;
-        const int ik = cmap[f.cmap_off[c] + k];
-        if (ik >= 0) a += Tv[fr[f.child[c]].T_off + ik];
+        const int64_t i = tsrc[f.cmap_off[c] + k];
+        if (i >= 0) a += Tv[i];
     }
     return a;
 }
@@ -1970,12 +1972,13 @@ __global__ __launch_bounds__(TPB) void k_nd_transpose(const int2 *__restrict__ w
 
 // forward solve of the fronts of one height: RPW consecutive rows k of M per
 // wave (4 waves per block), the lanes over the row's contiguous pivot columns
-// (j <= k on the lower-triangular pivot rows). z (the front's pivot right-hand side) is
-// formed in LDS by every block of the front.
+// (j <= k on the lower-triangular pivot rows). z (the front's pivot
+// right-hand side) is formed in LDS by every block of the front; bsrc holds
+// the level-vector index of each pivot.
 template <typename T, int RPW>
 __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
-                                                const int64_t *__restrict__ Pidx, const int *__restrict__ cmap,
-                                                const double *__restrict__ Mt, const int64_t *__restrict__ freeidx,
+                                                const int64_t *__restrict__ Pidx, const int64_t *__restrict__ bsrc,
+                                                const int64_t *__restrict__ tsrc, const double *__restrict__ Mt,
                                                 const T *__restrict__ bl, double *__restrict__ y,
                                                 double *__restrict__ Tv, const int *done) {
     if (done && *done) return;
@@ -1983,9 +1986,8 @@ __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, c
     const int2 w = work[blockIdx.x];
     const NdFront f = fr[w.x];
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
-    const int64_t *P = Pidx + f.P_off;
     for (int j = (int)threadIdx.x; j < f.p; j += TPB)
-        nd_sh[j] = (double)bl[freeidx[P[j]]] - nd_child_sum(f, fr, cmap, Tv, j);
+        nd_sh[j] = (double)bl[bsrc[f.P_off + j]] - nd_child_sum(f, tsrc, Tv, j);
     __syncthreads();
     for (int r = 0; r < RPW; ++r) {
         const int k = w.y + wv * RPW + r;      // RPW consecutive rows per wave
@@ -1997,8 +1999,8 @@ __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, c
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
         if (lane == 0) {
-            if (k < f.p) y[P[k]] = s;
-            else Tv[f.T_off + (k - f.p)] = s + nd_child_sum(f, fr, cmap, Tv, k);
+            if (k < f.p) y[Pidx[f.P_off + k]] = s;
+            else Tv[f.T_off + (k - f.p)] = s + nd_child_sum(f, tsrc, Tv, k);
         }
     }
 }
@@ -2008,7 +2010,7 @@ __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, c
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_nd_bwd(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
                                                 const int64_t *__restrict__ Pidx, const int64_t *__restrict__ Bidx,
-                                                const double *__restrict__ F, const int64_t *__restrict__ freeidx,
+                                                const double *__restrict__ F, const int64_t *__restrict__ bsrc,
                                                 const double *__restrict__ y, double *__restrict__ xc,
                                                 T *__restrict__ xl, const int *done) {
     if (done && *done) return;
@@ -2029,7 +2031,7 @@ __global__ __launch_bounds__(TPB) void k_nd_bwd(const int2 *__restrict__ work, c
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) {
         xc[P[j]] = s;
-        xl[freeidx[P[j]]] = (T)s;
+        xl[bsrc[f.P_off + j]] = (T)s;
     }
 }
 

@@ -100,7 +100,7 @@ struct tmg_solver {
     std::vector<int> nd_hstart;       // first front of each height (nd_nheight + 1)
     std::vector<int> nd_ea_start, nd_fw_start, nd_bw_start;   // work-list ranges per height
     NdFront *nd_fronts = nullptr;
-    int64_t *nd_P = nullptr, *nd_B = nullptr, *nd_src = nullptr, *nd_dst = nullptr;
+    int64_t *nd_P = nullptr, *nd_B = nullptr, *nd_src = nullptr, *nd_dst = nullptr, *nd_bsrc = nullptr, *nd_tsrc = nullptr;
     int *nd_cmap = nullptr, *nd_info = nullptr;
     int2 *nd_ea = nullptr, *nd_fw = nullptr, *nd_bw = nullptr, *nd_tp = nullptr;
     double *nd_F = nullptr, *nd_Mt = nullptr, *nd_T = nullptr, *nd_y = nullptr, *nd_xc = nullptr;
