@@ -73,7 +73,7 @@ void nd_post(const std::vector<NdNode> &t, int i, std::vector<int> &order) {
 
 // Symbolic analysis of the coarsest level (structure only: it holds for every
 // density). Returns false when the fronts do not fit the solve kernels' LDS.
-bool nd_symbolic(const DevLevel &lc, int leaf, int rpb, NdHost &o) {
+bool nd_symbolic(const DevLevel &lc, int leaf, NdHost &o) {
     const int nx = lc.nx, ny = lc.ny, nz = lc.nz;
     const int64_t nn = (int64_t)nx * ny * nz, nc = lc.nfree;
     std::vector<int64_t> cidx(3 * nn, -1);
@@ -217,7 +217,7 @@ bool nd_symbolic(const DevLevel &lc, int leaf, int rpb, NdHost &o) {
             const NdFront &F = o.fr[f];
             if (F.child[0] >= 0 || F.child[1] >= 0)
                 for (int l = 0; l < F.ld; ++l) o.ea.push_back(make_int2(f, l));
-            for (int r0 = 0; r0 < F.ld; r0 += rpb) o.fw.push_back(make_int2(f, r0));
+            for (int r0 = 0; r0 < F.ld; r0 += 4) o.fw.push_back(make_int2(f, r0));
             for (int j = 0; j < F.p; ++j) o.tp.push_back(make_int2(f, j));
             for (int j0 = 0; j0 < F.p; j0 += 4) o.bw.push_back(make_int2(f, j0));
         }
@@ -363,8 +363,7 @@ void nd_solve(tmg_solver *h, const T *b, T *x, const int *done) {
     const size_t sh = sizeof(double) * (size_t)h->nd_maxld;
     for (int hh = 0; hh < h->nd_nheight; ++hh) {
         const int w0 = h->nd_fw_start[hh], w1 = h->nd_fw_start[hh + 1];
-        auto kf = h->nd_rpb == 16 ? k_nd_fwd<T, 4> : k_nd_fwd<T, 1>;
-        hipLaunchKernelGGL(kf, dim3((unsigned)(w1 - w0)), dim3(TPB), sh, h->st, (const int2 *)h->nd_fw + w0,
+        hipLaunchKernelGGL((k_nd_fwd<T>), dim3((unsigned)(w1 - w0)), dim3(TPB), sh, h->st, (const int2 *)h->nd_fw + w0,
                            (const NdFront *)h->nd_fronts, (const int64_t *)h->nd_P, (const int64_t *)h->nd_bsrc,
                            (const int64_t *)h->nd_tsrc, (const double *)h->nd_Mt, b, h->nd_y, h->nd_T, done);
     }

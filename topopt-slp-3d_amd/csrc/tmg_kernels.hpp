@@ -1970,12 +1970,17 @@ __global__ __launch_bounds__(TPB) void k_nd_transpose(const int2 *__restrict__ w
     for (int k = (int)threadIdx.x; k < f.ld; k += TPB) out[(int64_t)k * f.p] = Mj[k];
 }
 
-// forward solve of the fronts of one height: RPW consecutive rows k of M per
-// wave (4 waves per block), the lanes over the row's contiguous pivot columns
-// (j <= k on the lower-triangular pivot rows). z (the front's pivot
-// right-hand side) is formed in LDS by every block of the front; bsrc holds
-// the level-vector index of each pivot.
-template <typename T, int RPW>
+// Each wave first loads its share of M into registers (up to ND_NREG values
+// per lane; longer rows or columns finish from memory), so that the matrix
+// loads overlap the dependent loads that gather the right-hand side.
+constexpr int ND_NREG = 16;
+
+// forward solve of the fronts of one height: one row k of M per wave (4 per
+// block), the lanes over the row's contiguous pivot columns (j <= k on the
+// lower-triangular pivot rows). z (the front's pivot right-hand side) is
+// formed in LDS by every block of the front; bsrc holds the level-vector
+// index of each pivot.
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, const NdFront *__restrict__ fr,
                                                 const int64_t *__restrict__ Pidx, const int64_t *__restrict__ bsrc,
                                                 const int64_t *__restrict__ tsrc, const double *__restrict__ Mt,
@@ -1986,22 +1991,26 @@ __global__ __launch_bounds__(TPB) void k_nd_fwd(const int2 *__restrict__ work, c
     const int2 w = work[blockIdx.x];
     const NdFront f = fr[w.x];
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const int k = w.y + wv;
+    const int jn = k >= f.ld ? 0 : (k < f.p ? k + 1 : f.p);
+    const double *Mk = Mt + f.Mt_off + (int64_t)k * f.p;
+    double m[ND_NREG];
+#pragma unroll
+    for (int c = 0; c < ND_NREG; ++c) m[c] = lane + 64 * c < jn ? Mk[lane + 64 * c] : 0.0;
     for (int j = (int)threadIdx.x; j < f.p; j += TPB)
         nd_sh[j] = (double)bl[bsrc[f.P_off + j]] - nd_child_sum(f, tsrc, Tv, j);
     __syncthreads();
-    for (int r = 0; r < RPW; ++r) {
-        const int k = w.y + wv * RPW + r;      // RPW consecutive rows per wave
-        if (k >= f.ld) return;
-        const int jn = k < f.p ? k + 1 : f.p;
-        const double *Mk = Mt + f.Mt_off + (int64_t)k * f.p;
-        double s = 0.0;
-        for (int j = lane; j < jn; j += 64) s += Mk[j] * nd_sh[j];
+    if (k >= f.ld) return;
+    double s = 0.0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) {
-            if (k < f.p) y[Pidx[f.P_off + k]] = s;
-            else Tv[f.T_off + (k - f.p)] = s + nd_child_sum(f, tsrc, Tv, k);
-        }
+    for (int c = 0; c < ND_NREG; ++c)
+        if (lane + 64 * c < jn) s += m[c] * nd_sh[lane + 64 * c];
+    for (int j = lane + 64 * ND_NREG; j < jn; j += 64) s += Mk[j] * nd_sh[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) {
+        if (k < f.p) y[Pidx[f.P_off + k]] = s;
+        else Tv[f.T_off + (k - f.p)] = s + nd_child_sum(f, tsrc, Tv, k);
     }
 }
 
@@ -2018,15 +2027,22 @@ __global__ __launch_bounds__(TPB) void k_nd_bwd(const int2 *__restrict__ work, c
     const int2 w = work[blockIdx.x];
     const NdFront f = fr[w.x];
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const int j = w.y + wv;
+    const int kn = j < f.p ? f.ld - j : 0;              // rows j .. ld-1 of column j
+    const double *Mj = F + f.F_off + (int64_t)j * f.ld + j;
+    double m[ND_NREG];
+#pragma unroll
+    for (int c = 0; c < ND_NREG; ++c) m[c] = lane + 64 * c < kn ? Mj[lane + 64 * c] : 0.0;
     const int64_t *P = Pidx + f.P_off;
     const int64_t *B = Bidx + f.B_off;
     for (int k = (int)threadIdx.x; k < f.ld; k += TPB) nd_sh[k] = k < f.p ? y[P[k]] : -xc[B[k - f.p]];
     __syncthreads();
-    const int j = w.y + wv;
     if (j >= f.p) return;
-    const double *Mj = F + f.F_off + (int64_t)j * f.ld;
     double s = 0.0;
-    for (int k = j + lane; k < f.ld; k += 64) s += Mj[k] * nd_sh[k];
+#pragma unroll
+    for (int c = 0; c < ND_NREG; ++c)
+        if (lane + 64 * c < kn) s += m[c] * nd_sh[j + lane + 64 * c];
+    for (int k = lane + 64 * ND_NREG; k < kn; k += 64) s += Mj[k] * nd_sh[j + k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) {

@@ -95,7 +95,6 @@ struct tmg_solver {
     // the coarse node grid instead of the dense inverse (tmg_coarse_nd.inc.hpp)
     bool coarse_nd = false;
     int nd_nfront = 0, nd_maxld = 0, nd_nheight = 0;
-    int nd_rpb = 4;                   // forward-solve rows per block (4 waves x 1 row, or 4 x 4)
     std::vector<NdFront> nd_fr;       // host copy of the front table (height order)
     std::vector<int> nd_hstart;       // first front of each height (nd_nheight + 1)
     std::vector<int> nd_ea_start, nd_fw_start, nd_bw_start;   // work-list ranges per height
@@ -1629,8 +1628,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (h->coarse_nd) {
         int leaf = 64;
         if (const char *ev = getenv("TMG_ND_LEAF")) leaf = std::max(1, atoi(ev));
-        if (const char *ev = getenv("TMG_ND_RPB")) h->nd_rpb = atoi(ev) == 16 ? 16 : 4;
-        if (!nd_symbolic(h->lev[h->L - 1], leaf, h->nd_rpb, ndh)) h->coarse_nd = false;
+        if (!nd_symbolic(h->lev[h->L - 1], leaf, ndh)) h->coarse_nd = false;
     }
     if (h->coarse_nd) h->coarse_symv = h->coarse_f32 = false;
     const int64_t ncd = h->coarse_nd ? 1 : h->nc;   // dense inverse size (per side)
