@@ -145,6 +145,7 @@ struct tmg_solver {
     bool wht_pf64 = true;             // spectral apply with the next plane prefetched: FP64 (faster)
     bool wht_pf32 = false;            // and FP32 (slower: the extra registers cost a wave)
     bool ebe_lds = true;              // SRC_X applies of the spectral form with the node plane staged through LDS (k_ebe_lds)
+    bool mr_lds = true;               // FP64 multiresolution fine level through k_ebe_lds as well (FP32: no gain)
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
     KWht<float> kw32{};
@@ -445,6 +446,12 @@ void launch_ebe(tmg_solver *h, const EbeArgs<T, TS> &a) {
                                    KWof<T>(h), h->xcd_wht);
                 return;
             }
+            if (std::is_same<T, double>::value && h->ebe_lds && h->mr_lds && h->nsub == 8) {
+                hipLaunchKernelGGL((k_ebe_lds<T, EPI, 8, 1>), gw, dim3(TPB), 0, h->st, a.nelx, a.nely, a.nelz, a.zchunk,
+                                   h->wht_tyl, h->wht_ntx, a.x, a.E, a.mask, a.b, a.invD, a.r, a.out, a.partial, a.done,
+                                   KWof<T>(h), h->xcd_wht);
+                return;
+            }
         }
         if (h->nsub == 8) {
             // multiresolution: one form per precision (FP64 prefetching)
@@ -557,6 +564,14 @@ void apply(tmg_solver *h, int l, const T *x, const T *b, T *out, double *partial
 // level, the masked apply gives the same free-DOF values.
 void apply_pcg_q(tmg_solver *h, bool consumer_masks, const int *done) {
     const DevLevel &l0 = h->lev[0];
+    if (consumer_masks && h->wht && h->ebe_lds && h->mr_lds && h->nsub == 8 && !h->fine_stencil) {
+        dim3 gw((unsigned)(h->wht_ntx * h->wht_nty), (unsigned)((l0.nz + h->zchunk64 - 1) / h->zchunk64));
+        hipLaunchKernelGGL((k_ebe_lds<double, EPI_DOTU, 8, 1>), gw, dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz,
+                           h->zchunk64, h->wht_tyl, h->wht_ntx, (const double *)h->pd, (const double *)h->E64, l0.mask,
+                           (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, h->pq, h->partial,
+                           done, (const double *)h->kw64_d, h->xcd_wht);
+        return;
+    }
     if (consumer_masks && h->wht && h->ebe_lds && h->nsub == 1 && !h->fine_stencil) {
         dim3 gw((unsigned)(h->wht_ntx * h->wht_nty), (unsigned)((l0.nz + h->zchunk64 - 1) / h->zchunk64));
         hipLaunchKernelGGL((k_ebe_lds<double, EPI_DOTU, 1, 3>), gw, dim3(TPB), 0, h->st, l0.nelx, l0.nely, l0.nelz,
@@ -1407,6 +1422,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_ST3_MAXNODES")) h->st3_maxnodes = atoll(ev);
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
+    if (const char *ev = getenv("TMG_MR_LDS")) h->mr_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
