@@ -104,6 +104,12 @@ struct tmg_solver {
     int2 *nd_ea = nullptr, *nd_fw = nullptr, *nd_bw = nullptr, *nd_tp = nullptr;
     double *nd_F = nullptr, *nd_Mt = nullptr, *nd_T = nullptr, *nd_y = nullptr, *nd_xc = nullptr;
     int64_t nd_nent = 0, nd_Fsize = 0, nd_ntp = 0;
+    // the fronts of one height are factorised on up to 4 side streams, each
+    // with its own rocBLAS handle (handles that own their stream only)
+    int nd_ns = 0;
+    hipStream_t nd_st[4] = {nullptr, nullptr, nullptr, nullptr};
+    rocblas_handle nd_rb[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t nd_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     rocblas_handle rb = nullptr;
     // PCG
     double *px = nullptr, *pr = nullptr, *pd = nullptr, *pd2 = nullptr, *pq = nullptr, *pf = nullptr;
@@ -1642,7 +1648,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_COARSE_ND")) h->coarse_nd = ev[0] != '0';
     NdHost ndh;
     if (h->coarse_nd) {
-        int leaf = 64;
+        int leaf = 256;
         if (const char *ev = getenv("TMG_ND_LEAF")) leaf = std::max(1, atoi(ev));
         if (!nd_symbolic(h->lev[h->L - 1], leaf, ndh)) h->coarse_nd = false;
     }
