@@ -515,26 +515,27 @@ def test_nested_dissection_coarse_solve_matches_dense_and_oracle(oracle_mod, mon
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
 
 
-@pytest.mark.parametrize("leaf", [8, 27])
-def test_nested_dissection_with_roller_supports(oracle_mod, monkeypatch, leaf):
+@pytest.mark.parametrize("leaf,cycle", [(8, 0), (27, 0), (8, 1), (27, 2)])
+def test_nested_dissection_with_roller_supports(oracle_mod, monkeypatch, leaf, cycle):
     """Nested dissection on a quarter-symmetry beam whose supports fix single
     displacement components (symmetry planes x = 0 and z = 0, a roller line):
-    coarse nodes with one or two free DOFs. The V-cycle equals the dense
-    solve's to rounding; the PCG solve meets the oracle."""
+    coarse nodes with one or two free DOFs, under the V, W and FMV cycles.
+    The preconditioner equals the dense solve's to rounding; the PCG solve
+    meets the oracle's with the same cycle."""
     nelx, nely, nelz, L = 32, 16, 16, 3
-    _say(f"nested dissection, roller supports, leaf={leaf}")
+    _say(f"nested dissection, roller supports, leaf={leaf}, cycle={cycle}")
     supp = [[0, 0, 0, nely, 0, nelz, 1, 0, 0], [0, nelx, 0, nely, 0, 0, 0, 0, 1], [nelx, nelx, 0, 0, 0, nelz, 0, 1, 0]]
     prob = P.from_rows(nelx, nely, nelz, supp, [[0, 0, nely, nely, 0, nelz, 0, -1, 0]])
     x = P.random_density(nelx, nely, nelz, 35)
     o = _oracle(oracle_mod, prob, x, L)
     monkeypatch.setenv("TMG_COARSE_ND", "0")
-    s0 = _setup(prob, x, L, "fp64")
+    s0 = _setup(prob, x, L, "fp64", cycle=cycle)
     monkeypatch.setenv("TMG_COARSE_ND", "1")
     monkeypatch.setenv("TMG_ND_LEAF", str(leaf))
-    s1 = _setup(prob, x, L, "fp64")
+    s1 = _setup(prob, x, L, "fp64", cycle=cycle)
     b = np.random.default_rng(35).standard_normal(prob.nfree)
     assert _rel(s1.vcycle(b), s0.vcycle(b)) < 1e-12
-    u_or, it_or, _ = o.solve(prob.f)
+    u_or, it_or, _ = o.solve(prob.f, cycle=cycle)
     u, it, rr, rc = s1.solve(prob.f)
     assert rc == 0 and rr <= 1e-8 and abs(it - it_or) <= 1
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
