@@ -9,6 +9,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
@@ -1796,8 +1797,10 @@ int tmg_update_density_device(tmg_solver *h, const double *d_x) {
 int tmg_update_density(tmg_solver *h, const double *x) {
     if (!h || !x) return TMG_ERR_ARG;
     const int64_t n = h->lev[0].ne * h->nsub;
-    for (int64_t e = 0; e < n; ++e)
-        if (!(x[e] >= 0.0) || !std::isfinite(x[e])) return set_err(h, TMG_ERR_ARG, "densities must be finite and >= 0");
+    // branch-free scan (vectorised): NaN fails both comparisons, +inf the second
+    bool ok = true;
+    for (int64_t e = 0; e < n; ++e) ok &= (x[e] >= 0.0) & (x[e] <= DBL_MAX);
+    if (!ok) return set_err(h, TMG_ERR_ARG, "densities must be finite and >= 0");
     HIPX(hipMemcpyAsync(h->xdens, host_src(h, x, n), sizeof(double) * n, hipMemcpyHostToDevice, h->st));
     return refresh(h);
 }

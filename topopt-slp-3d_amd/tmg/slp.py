@@ -129,6 +129,37 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
     else:
         fixed = np.zeros(nelem, dtype=bool) if fixed_design is None else np.asarray(fixed_design, dtype=bool)
     freeD = ~fixed
+    anyfixed = bool(fixed.any())
+
+    def free(v):
+        """v on the free design variables (v itself when none is fixed)."""
+        return v[freeD] if anyfixed else v
+
+    def box_gap(xv, g):
+        """max |min(1, max(0, x - g)) - x| over the free design variables
+        (StrSLP.m:247-256, 482-486), with one temporary."""
+        xf = free(xv)
+        if xf.size == 0:
+            return 0.0
+        t = xf - g
+        np.maximum(t, 0.0, out=t)
+        np.minimum(t, 1.0, out=t)
+        t -= xf
+        np.abs(t, out=t)
+        return float(t.max())
+
+    def trust_box(d, scale=1.0):
+        """Box bounds of the step: max(-d, -x) <= s <= min(d, 1 - x), 0 on
+        fixed design variables (StrSLP.m:234-238)."""
+        lo = np.negative(x)
+        np.maximum(lo, -scale * d, out=lo)
+        hi = np.subtract(1.0, x)
+        np.minimum(hi, scale * d, out=hi)
+        if anyfixed:
+            lo[fixed] = 0.0
+            hi[fixed] = 0.0
+        return lo, hi
+
     # str.freeDensG = str.freeDens: fixed elements carry no gradient (StrSLP.m:75, Grad.m:18-23)
     freedens = freeD.astype(np.uint8) if (fixed.any() and n == 1) else None
 
@@ -188,16 +219,12 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
     # volume row and its right-hand side (StrSLP.m:226-233)
     A = (apply_filter(velem, True) if opfilter != 0 else velem) / Vmax
     b = (velem @ xfil - Vmax) / Vmax
-    sL = np.maximum(-delta, -x)
-    sU = np.minimum(delta, 1.0 - x)
-    sL[fixed] = 0.0
-    sU[fixed] = 0.0
+    sL, sU = trust_box(delta)
 
     # optimality of the initial point (StrSLP.m:247-256)
     lam = 0.0
     F = Fold
-    gradpbox = np.minimum(1, np.maximum(0.0, x[freeD] - Fgrad[freeD])) - x[freeD]
-    gpnorm = float(np.abs(gradpbox).max()) if gradpbox.size else 0.0
+    gpnorm = box_gap(x, free(Fgrad))
     if gpnorm < 1e-12 and b < 1e-12 and (x - 1.0).max() < 1e-12 and x.min() > -1e-12:
         countG = countF = slp.maxcount
 
@@ -209,10 +236,7 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
         if not ok:
             # infeasible trust-region LP: minimise the volume violation instead (StrSLP.m:271-285)
             flag = -2
-            sL2 = np.maximum(-0.8 * delta, -x)
-            sU2 = np.minimum(0.8 * delta, 1.0 - x)
-            sL2[fixed] = 0.0
-            sU2[fixed] = 0.0
+            sL2, sU2 = trust_box(delta, 0.8)
             s, lam = feasibility_step(A, b, sL2, sU2, equality=not volineq)
         fval = float(Fgrad @ s)
         tm["LP"] += time.perf_counter() - t
@@ -258,18 +282,15 @@ def StrSLP(prob, volfrac, p=3.0, emin=1e-9, opfilter=2, rmin=1.5, volineq=True, 
             Fold = F
             u_free = u_new
             Fgrad = gradient(prob.full(u_free), xfil)
-            Lgrad = Fgrad[freeD] + lam * A[freeD]
-            gradpbox = np.minimum(1, np.maximum(0.0, x[freeD] - Lgrad)) - x[freeD]
-            gpnorm = float(np.abs(gradpbox).max()) if gradpbox.size else 0.0
+            Lgrad = free(A) * lam
+            Lgrad += free(Fgrad)
+            gpnorm = box_gap(x, Lgrad)
             iter_ += 1
             countG = countG + 1 if gpnorm <= slp.tolG else 0
             countF = countF + 1 if ared <= slp.tolF else 0
         countS = countS + 1 if snorm <= slp.tolS else 0
 
-        sL = np.maximum(-delta, -x)
-        sU = np.minimum(delta, 1.0 - x)
-        sL[fixed] = 0.0
-        sU[fixed] = 0.0
+        sL, sU = trust_box(delta)
         rec = dict(it=itno, F=F, ared=ared, pred=pred, snorm=snorm, delta=deltaold, gpnorm=gpnorm, flag=flag,
                    pcgit=it, accepted=bool(accepted), lam=lam)
         history.append(rec)
