@@ -1588,18 +1588,55 @@ __global__ __launch_bounds__(TPB) void k_xr_update(int64_t n, double *__restrict
     else block_reduce_store(s, partial, blockIdx.x);
 }
 
+// Fixed-order total of np partial sums by one block of TPB threads, the
+// same order as k_finalize's (strided per-thread sums, wave shuffles, the
+// wave sums in order); every thread gets the total. A consumer kernel that
+// needs a PCG scalar computes it this way in every block, instead of waiting
+// for a one-block k_finalize launch.
+__device__ inline double fin_block_total(int np, const double *__restrict__ partial) {
+    __shared__ double red[TPB / 64];
+    __shared__ double tot;
+    double s = 0.0;
+    for (int i = (int)threadIdx.x; i < np; i += TPB) s += partial[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) t += red[w];
+        tot = t;
+    }
+    __syncthreads();
+    return tot;
+}
+
 // The PCG's vector passes with the solution update deferred by one iteration
 // (x_{k+1} = x_k + alpha_k d_k is the same sum, done when d_k is read anyway):
 // k_r_update: r -= alpha q; partial ||r||^2; and the next preconditioner
 // cycle's first Jacobi sweep from zero, x0 = invD r (in its precision TX).
 // invD is 0 exactly at the fixed DOFs (and positive elsewhere), so r stays 0
 // there whatever q holds: q may come from the unmasked apply (EPI_DOTU).
-template <typename TX>
+// FUSE: alpha = rho / (d'q) from the apply's npa partial sums pa in every
+// block (block 0 stores dq and alpha); the block sums go to partial, which
+// must not be pa.
+template <typename TX, bool FUSE = false>
 __global__ __launch_bounds__(TPB) void k_r_update(int64_t n, double *__restrict__ r, const double *__restrict__ q,
                                                   const TX *__restrict__ invD, TX *__restrict__ x0, double *partial,
-                                                  PcgScalars *sc) {
+                                                  PcgScalars *sc, int npa = 0, const double *pa = nullptr) {
     if (sc->done) return;
-    const double al = sc->alpha;
+    double al;
+    if constexpr (FUSE) {
+        const double dq = fin_block_total(npa, pa);
+        al = sc->rho / dq;                      // the FIN_ALPHA recurrence
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->dq = dq;
+            sc->alpha = al;
+        }
+    } else {
+        al = sc->alpha;
+    }
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
         const TX iv = invD[i];
@@ -1613,16 +1650,32 @@ __global__ __launch_bounds__(TPB) void k_r_update(int64_t n, double *__restrict_
 
 // k_dir_xupdate: x += alpha d with the previous iteration's alpha and d (0
 // before the first iteration: alpha = 0 and d = 0 at the start of a solve),
-// then d = s + beta d (d = s on a restart iteration).
-template <typename TS>
+// then d = s + beta d (d = s on a restart iteration). FUSE: rho = r's from
+// the post-sweep's npr partial sums pr and beta = rho / gamma in every block
+// (block 0 stores rho and beta; gamma, it and restart are not written here).
+template <typename TS, bool FUSE = false>
 __global__ __launch_bounds__(TPB) void k_dir_xupdate(int64_t n, const TS *__restrict__ s, double *__restrict__ d,
-                                                     double *__restrict__ x, const PcgScalars *sc) {
+                                                     double *__restrict__ x, PcgScalars *sc, int npr = 0,
+                                                     const double *pr = nullptr) {
     if (sc->done) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
-    const double di = d[i];
-    x[i] = x[i] + sc->alpha * di;
-    d[i] = (sc->it == sc->restart) ? (double)s[i] : sc->beta * di + (double)s[i];
+    const bool restart = sc->it == sc->restart;
+    double beta;
+    if constexpr (FUSE) {
+        const double rho = fin_block_total(npr, pr);
+        beta = restart ? 0.0 : rho / sc->gamma;  // the FIN_RHO recurrence
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rho = rho;
+            sc->beta = beta;
+        }
+    } else {
+        beta = sc->beta;
+    }
+    const double al = sc->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        const double di = d[i];
+        x[i] = x[i] + al * di;
+        d[i] = restart ? (double)s[i] : beta * di + (double)s[i];
+    }
 }
 
 // The deferred update of the last iteration, once the PCG has stopped.

@@ -153,6 +153,7 @@ struct tmg_solver {
     bool wht_pf32 = false;            // and FP32 (slower: the extra registers cost a wave)
     bool ebe_lds = true;              // SRC_X applies of the spectral form with the node plane staged through LDS (k_ebe_lds)
     bool mr_lds = true;               // FP64 multiresolution fine level through k_ebe_lds as well (FP32: no gain)
+    bool fin_consumer = true;         // rho/beta and alpha reduced in their consumers' blocks (no k_finalize<1,2>)
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
     KWht<float> kw32{};
@@ -854,6 +855,10 @@ void pcg_iteration(tmg_solver *h, int parity) {
     int nrho = 0;
     const bool xd = xdefer_ok(h);
     precondition(h, h->pr, &s64, &s32, done, &nrho, xd);
+    // rho/beta and alpha computed by every block of their consumers
+    // (k_dir_xupdate, k_r_update) instead of one-block k_finalize launches
+    const bool fc = xd && h->fin_consumer && !h->fin_fused && nrho > 0 && !fused_ok(h);
+    const unsigned gfc = (unsigned)std::min<int64_t>(gdot, 2048);
     if (fused_ok(h)) {
         // rho = r' s (partials from the post-smoothing apply, preconditioner precision)
         const int npost = (int)apply_blocks(h, 0, s32 == nullptr);
@@ -867,7 +872,7 @@ void pcg_iteration(tmg_solver *h, int parity) {
     } else {
         if (nrho > 0) {
             // rho = r' s from the post-sweep's partial sums (finished there when fin_fused)
-            if (!h->fin_fused)
+            if (!h->fin_fused && !fc)
                 hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, nrho, h->partial, 0, nullptr,
                                    h->d_sc);
         } else {
@@ -875,9 +880,16 @@ void pcg_iteration(tmg_solver *h, int parity) {
             else hipLaunchKernelGGL((k_dot<double, double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, s64, h->partial, done);
             hipLaunchKernelGGL(k_finalize<FIN_RHO>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
         }
-        if (xd) {
-            if (s32) hipLaunchKernelGGL((k_dir_xupdate<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->px, h->d_sc);
-            else hipLaunchKernelGGL((k_dir_xupdate<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->px, h->d_sc);
+        if (fc) {
+            if (s32)
+                hipLaunchKernelGGL((k_dir_xupdate<float, true>), dim3(gfc), dim3(TPB), 0, h->st, n, s32, h->pd, h->px,
+                                   h->d_sc, nrho, (const double *)h->partial);
+            else
+                hipLaunchKernelGGL((k_dir_xupdate<double, true>), dim3(gfc), dim3(TPB), 0, h->st, n, s64, h->pd, h->px,
+                                   h->d_sc, nrho, (const double *)h->partial);
+        } else if (xd) {
+            if (s32) hipLaunchKernelGGL((k_dir_xupdate<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->px, h->d_sc, 0, (const double *)nullptr);
+            else hipLaunchKernelGGL((k_dir_xupdate<double>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s64, h->pd, h->px, h->d_sc, 0, (const double *)nullptr);
         } else if (s32) {
             hipLaunchKernelGGL((k_dir_update<float>), dim3(nblk(n)), dim3(TPB), 0, h->st, n, s32, h->pd, h->d_sc);
         } else {
@@ -893,23 +905,36 @@ void pcg_iteration(tmg_solver *h, int parity) {
         } else {
             apply_pcg_q(h, xd, done);      // q = A d, d'q
             kt_mark(h, KT_APPLY, true);
-            hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0, nullptr,
-                               h->d_sc);
+            if (!fc)
+                hipLaunchKernelGGL(k_finalize<FIN_ALPHA>, dim3(1), dim3(FIN_THREADS), 0, h->st, napply, h->partial, 0,
+                                   nullptr, h->d_sc);
         }
-        if (xd) {
+        if (fc) {
+            // ||r||^2 partials into partial2: the blocks still read the apply's partials
+            DevLevel &l0 = h->lev[0];
+            if (h->pprec == TMG_MIXED)
+                hipLaunchKernelGGL((k_r_update<float, true>), dim3(gfc), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv32,
+                                   (float *)l0.x, h->partial2, h->d_sc, napply, (const double *)h->partial);
+            else
+                hipLaunchKernelGGL((k_r_update<double, true>), dim3(gfc), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv64,
+                                   (double *)l0.x, h->partial2, h->d_sc, napply, (const double *)h->partial);
+        } else if (xd) {
             DevLevel &l0 = h->lev[0];
             if (h->pprec == TMG_MIXED)
                 hipLaunchKernelGGL((k_r_update<float>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv32,
-                                   (float *)l0.x, h->partial, h->d_sc);
+                                   (float *)l0.x, h->partial, h->d_sc, 0, (const double *)nullptr);
             else
                 hipLaunchKernelGGL((k_r_update<double>), dim3(gdot), dim3(TPB), 0, h->st, n, h->pr, h->pq, l0.inv64,
-                                   (double *)l0.x, h->partial, h->d_sc);
+                                   (double *)l0.x, h->partial, h->d_sc, 0, (const double *)nullptr);
         } else {
             hipLaunchKernelGGL(k_xr_update, dim3(gdot), dim3(TPB), 0, h->st, n, h->px, h->pr, h->pd, h->pq, h->partial,
                                h->d_sc, finrr);
         }
     }
-    if (!h->fin_fused)
+    if (fc)
+        hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gfc, h->partial2, 0, nullptr,
+                           h->d_sc);
+    else if (!h->fin_fused)
         hipLaunchKernelGGL(k_finalize<FIN_RR>, dim3(1), dim3(FIN_THREADS), 0, h->st, (int)gdot, h->partial, 0, nullptr, h->d_sc);
 }
 
@@ -1430,6 +1455,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_EBE_PREFETCH")) h->ebe_prefetch = ev[0] != '0';
     if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_MR_LDS")) h->mr_lds = ev[0] != '0';
+    if (const char *ev = getenv("TMG_FIN_CONSUMER")) h->fin_consumer = ev[0] != '0';
     if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
