@@ -251,8 +251,10 @@ int tmg_server_wait(tmg_server *s, int64_t id, double timeout_s, int *state);
  * times[4] = {arrival, admission, start of PCG, completion} on the server clock. */
 int tmg_server_result(tmg_server *s, int64_t id, double *u, int *it, double *relres, double *times);
 int tmg_server_release(tmg_server *s, int64_t id);
-/* stats[7] = {bytes reserved, peak bytes reserved, queued, running, done, clock,
- * peak number of requests solving at the same time}. */
+/* stats[8] = {bytes reserved, peak bytes reserved, queued, running, done, clock,
+ * peak number of requests solving at the same time, number of requests whose
+ * hierarchy was rebuilt after a failed first coarse factorization with their
+ * densities verified intact on the device}. */
 int tmg_server_stats(tmg_server *s, double *stats);
 double tmg_server_clock(const tmg_server *s);
 /* Device bytes the server reserves for a request of this shape. */

@@ -32,7 +32,8 @@ def test_serving_stream_256_requests(oracle_mod):
     srv.close()
     summ = serve.summarize(recs)
     _say(f"  p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms, {summ['requests_per_s']:.1f} requests/s, "
-         f"peak running {st['peak_running']}, peak reserved {st['peak_bytes_reserved'] / 1e9:.2f} GB")
+         f"peak running {st['peak_running']}, peak reserved {st['peak_bytes_reserved'] / 1e9:.2f} GB, "
+         f"rebuilt after a failed first factorization {st['rebuilt']}")
     for i, err in summ.get("failed_requests", []):
         _say(f"  request {i} failed: {err}")
     assert summ["failed"] == 0 and summ["all_converged"]

@@ -112,6 +112,7 @@ struct tmg_server {
     size_t budget = 0, used = 0, peak = 0;
     int nslots = 0;
     int peak_running = 0;          // most requests solving at the same time
+    int rebuilds = 0;              // operators rebuilt after a failed first coarse factorization
     // per slot: stream, pinned scalar block, pinned staging for the
     // request's host vectors, and its own stream-ordered memory pool (device
     // memory is reused only within a slot, i.e. only on the slot's stream)
@@ -170,19 +171,28 @@ int srv_verify_upload(tmg_solver *h, const double *d, const double *hv, int64_t 
     return 0;
 }
 
-// A failed coarse factorisation is reported with what tells a corrupted
+// A failed coarse factorisation is examined for what tells a corrupted
 // set-up from a singular operator: whether the device still holds the
 // request's densities, and whether building the same hierarchy again on the
-// slot's stream, with nothing else in flight on it, factorises.
-void srv_diagnose_not_spd(tmg_server *S, tmg_solver *h, SrvReq *r) {
+// slot's stream, with nothing else in flight on it, factorises. If the
+// densities are intact and the rebuild factorises, the request goes on with
+// the rebuilt hierarchy (every level and the factor are recomputed from the
+// densities), the event is printed and counted (tmg_server_stats); otherwise
+// the request fails with both messages. Returns the request's status.
+int srv_diagnose_not_spd(tmg_server *S, tmg_solver *h, SrvReq *r) {
     std::string first = h->err;
     uint64_t dev = 0;
-    std::string dens = device_checksum(h, h->xdens, (int64_t)r->x.size(), &dev) == 0
-                           ? (dev == host_checksum(r->x.data(), (int64_t)r->x.size()) ? "intact" : "corrupted")
-                           : "unreadable";
-    const int rc2 = refresh(h);
+    const bool readable = device_checksum(h, h->xdens, (int64_t)r->x.size(), &dev) == 0;
+    const bool intact = readable && dev == host_checksum(r->x.data(), (int64_t)r->x.size());
+    const std::string dens = readable ? (intact ? "intact" : "corrupted") : "unreadable";
+    const int rc2 = intact ? refresh(h) : TMG_ERR_NOT_SPD;
     h->err = first + "; densities on the device " + dens + "; rebuilding the same hierarchy " +
              (rc2 == 0 ? "factorised (the first build was corrupted)" : "failed again: " + h->err);
+    if (rc2 != 0) return TMG_ERR_NOT_SPD;
+    fprintf(stderr, "tmg_server: request %lld: %s\n", (long long)r->id, h->err.c_str());
+    std::lock_guard<std::mutex> lk(S->mu);
+    ++S->rebuilds;
+    return 0;
 }
 
 // The request's arena goes back to its slot's pool, ordered after the
@@ -242,7 +252,7 @@ tmg_solver *srv_setup(tmg_server *S, SrvReq *r, rocblas_handle rb, int *rc, std:
     *rc = tmg_update_density(h, r->x.data());   // each worker's own rocBLAS handle and stream
     const double tp2 = S->now();
     if (!*rc) *rc = srv_verify_upload(h, h->xdens, r->x.data(), (int64_t)r->x.size(), "densities");
-    if (*rc == TMG_ERR_NOT_SPD) srv_diagnose_not_spd(S, h, r);
+    if (*rc == TMG_ERR_NOT_SPD) *rc = srv_diagnose_not_spd(S, h, r);
     if (!*rc && (*rc = upload_free(h, 0, r->x0.data(), h->px)) == 0) *rc = upload_free(h, 0, r->f.data(), h->pf);
     h->have_rhs = true;
     const double tp3 = S->now();
@@ -482,6 +492,7 @@ int tmg_server_stats(tmg_server *S, double *stats) {
     stats[4] = (double)done;
     stats[5] = S->now();
     stats[6] = (double)S->peak_running;
+    stats[7] = (double)S->rebuilds;
     return TMG_OK;
 }
 

@@ -193,6 +193,7 @@ struct tmg_solver {
     size_t bytes = 0;                 // device bytes held by this handle now
     size_t peak_bytes = 0;            // and at most, over its lifetime
     std::unordered_map<void *, size_t> sizes;   // rounded size of each live allocation
+    std::unordered_map<void *, std::pair<size_t, int>> canary;   // TMG_DEBUG_CANARY: data bytes, allocating line
     bool have_op = false;
     bool have_rhs = false;            // pf holds a right-hand side (a solve was started)
     // kernel timing inside a solve (tmg_time_in_solve): event pairs recorded
@@ -246,15 +247,27 @@ bool poison_buffers() {
     return on;
 }
 
+// TMG_DEBUG_CANARY=1: every device buffer is followed by 4 KiB of 0xA5
+// bytes, checked after set-up, each density update and each solve, so that
+// a write past the end of a buffer is reported with the source line of the
+// buffer's allocation (instead of landing in a neighbouring buffer)
+constexpr size_t CANARY_BYTES = 4096;
+bool canary_buffers() {
+    static const bool on = getenv_flag("TMG_DEBUG_CANARY");
+    return on;
+}
+
 template <typename T>
-int dalloc(tmg_solver *h, T **p, int64_t n) {
+int dalloc(tmg_solver *h, T **p, int64_t n, int line = __builtin_LINE()) {
     const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    const size_t guard = canary_buffers() ? CANARY_BYTES : 0;
+    const size_t total = round256(bytes) + guard;
     hipError_t e = hipSuccess;
-    if (h->arena && h->arena_off + round256(bytes) <= h->arena_cap) {
+    if (h->arena && h->arena_off + total <= h->arena_cap) {
         *p = reinterpret_cast<T *>(h->arena + h->arena_off);
-        h->arena_off += round256(bytes);
+        h->arena_off += total;
     } else {
-        e = h->pool ? hipMallocFromPoolAsync((void **)p, bytes, h->pool, h->st) : hipMalloc((void **)p, bytes);
+        e = h->pool ? hipMallocFromPoolAsync((void **)p, total, h->pool, h->st) : hipMalloc((void **)p, total);
     }
     if (e != hipSuccess) {
         *p = nullptr;
@@ -262,9 +275,37 @@ int dalloc(tmg_solver *h, T **p, int64_t n) {
         return TMG_ERR_OOM;
     }
     if (poison_buffers() && h->st) (void)hipMemsetAsync((void *)*p, 0xFF, bytes, h->st);
-    h->bytes += round256(bytes);
+    if (guard) {
+        (void)hipMemsetAsync(reinterpret_cast<char *>(*p) + round256(bytes), 0xA5, guard, h->st);
+        h->canary[(void *)*p] = std::make_pair(round256(bytes), line);
+    }
+    h->bytes += total;
     h->peak_bytes = std::max(h->peak_bytes, h->bytes);
-    h->sizes[(void *)*p] = round256(bytes);
+    h->sizes[(void *)*p] = total;
+    return 0;
+}
+
+// reports (and returns TMG_ERR_HIP for) the first buffer whose guard bytes changed
+int canary_check(tmg_solver *h, const char *where) {
+    if (!canary_buffers() || h->canary.empty()) return 0;
+    if (hipStreamSynchronize(h->st) != hipSuccess) return 0;
+    std::vector<uint8_t> g(CANARY_BYTES);
+    for (const auto &kv : h->canary) {
+        // stream-ordered: other threads may be capturing graphs
+        if (hipMemcpyAsync(g.data(), reinterpret_cast<char *>(kv.first) + kv.second.first, CANARY_BYTES,
+                           hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+            hipStreamSynchronize(h->st) != hipSuccess)
+            continue;
+        size_t bad = 0;
+        while (bad < CANARY_BYTES && g[bad] == 0xA5) ++bad;
+        if (bad < CANARY_BYTES) {
+            h->err = std::string("guard bytes overwritten ") + where + ": buffer of " + std::to_string(kv.second.first) +
+                     " bytes allocated at tmg_solver.hip line " + std::to_string(kv.second.second) + ", first changed byte " +
+                     std::to_string(bad) + " past its end";
+            fprintf(stderr, "tmg: %s\n", h->err.c_str());
+            return TMG_ERR_HIP;
+        }
+    }
     return 0;
 }
 
@@ -285,6 +326,7 @@ void dfree(tmg_solver *h, T *&p) {
             h->bytes -= it->second;
             h->sizes.erase(it);
         }
+        h->canary.erase((void *)c);
     }
     p = nullptr;
 }
@@ -1062,10 +1104,11 @@ int solve_result(tmg_solver *h, int *it, double *relres) {
 
 // After the PCG has stopped: the last iteration's deferred solution update.
 int solve_finish(tmg_solver *h) {
-    if (!xdefer_ok(h)) return 0;
+    if (!xdefer_ok(h)) return canary_check(h, "by the solve");
     const int64_t n = h->lev[0].ndof;
     hipLaunchKernelGGL(k_x_final, dim3(nblk(n)), dim3(TPB), 0, h->st, n, h->px, h->pd, h->d_sc);
-    return check_launch(h);
+    TRY(check_launch(h));
+    return canary_check(h, "by the solve");
 }
 
 // Blocking device-resident solve on px (x0 in, x out) and pf.
@@ -1275,6 +1318,7 @@ int refresh(tmg_solver *h) {
     if (h->coarse_nd) {
         TRY(nd_factor(h));
         HIPX(hipStreamSynchronize(h->st));
+        TRY(canary_check(h, "by the density update"));
         h->have_op = true;
         return 0;
     }
@@ -1308,6 +1352,7 @@ int refresh(tmg_solver *h) {
     }
     TRY(check_launch(h));
     HIPX(hipStreamSynchronize(h->st));
+    TRY(canary_check(h, "by the density update"));
     h->have_op = true;
     return 0;
 }
@@ -1671,7 +1716,9 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_COARSE_SYMV")) h->coarse_symv = ev[0] != '0';
     // large coarse systems: nested dissection and a multifrontal factorization
     // (falls back to the dense inverse when its fronts do not fit the solve)
-    h->coarse_nd = h->nc >= 16384;
+    int64_t nd_min = 16384;
+    if (const char *ev = getenv("TMG_ND_MIN_DOFS")) nd_min = std::max<int64_t>(1, atoll(ev));
+    h->coarse_nd = h->nc >= nd_min;
     if (const char *ev = getenv("TMG_COARSE_ND")) h->coarse_nd = ev[0] != '0';
     NdHost ndh;
     if (h->coarse_nd) {
@@ -1798,6 +1845,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
         }
         rocblas_set_stream(h->rb, h->st);
     }
+    if ((rc = canary_check(h, "by the set-up")) != 0) return fail(rc);
     *out = h;
     return TMG_OK;
 }

@@ -85,10 +85,10 @@ class Server:
         self._meta.pop(rid, None)
 
     def stats(self):
-        s = (C.c_double * 7)()
+        s = (C.c_double * 8)()
         lib().tmg_server_stats(self.h, s)
         return {"bytes_reserved": s[0], "peak_bytes_reserved": s[1], "queued": int(s[2]), "running": int(s[3]),
-                "done": int(s[4]), "clock": s[5], "peak_running": int(s[6])}
+                "done": int(s[4]), "clock": s[5], "peak_running": int(s[6]), "rebuilt": int(s[7])}
 
     def clock(self):
         return lib().tmg_server_clock(self.h)
@@ -209,5 +209,6 @@ def benchmark(reqs, budget_bytes=200e9, max_active=16, cap_sample=32, load=0.8, 
     out = summarize(recs)
     out.update(single_stream_capacity_rps=cap, arrival_rate_rps=rate, load=load, max_active=max_active,
                budget_bytes=budget_bytes, peak_bytes_reserved=st["peak_bytes_reserved"],
+               rebuilt_after_failed_factorization=st["rebuilt"],
                mean_serial_service_ms=1e3 * float(np.mean(serial)))
     return out
