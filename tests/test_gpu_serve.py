@@ -50,6 +50,36 @@ def test_same_alone_and_in_the_mix():
     assert st["peak_running"] >= 2
 
 
+def test_rebuild_after_a_failed_first_factorization(monkeypatch):
+    """A request whose first coarse factorization fails (forced from the host
+    with TMG_DEBUG_FAIL_FIRST_FACTOR: every kernel runs normally, the host then
+    reports the failure) and whose densities check intact is rebuilt once and
+    counted; its result equals that of an undisturbed server, bit for bit."""
+    from tmg import serve
+    reqs = _requests(4, seed=11)
+
+    def run():
+        srv = serve.Server(budget_bytes=8e9, max_active=4)
+        ids = [srv.submit(r["prob"], r["x"], r["ngrids"]) for r in reqs]
+        out = []
+        for rid in ids:
+            assert srv.wait(rid, 120)[0]
+            out.append(srv.result(rid))
+        st = srv.stats()
+        srv.close()
+        return out, st
+
+    _say("undisturbed")
+    ref, st0 = run()
+    _say("first factorization of every request reported failed")
+    monkeypatch.setenv("TMG_DEBUG_FAIL_FIRST_FACTOR", "1")
+    got, st1 = run()
+    assert st0["rebuilt"] == 0 and st1["rebuilt"] == len(reqs)
+    for a, b in zip(ref, got):
+        assert a["rc"] == 0 and b["rc"] == 0 and a["it"] == b["it"]
+        assert np.array_equal(a["u"], b["u"])
+
+
 def test_server_matches_direct_solver():
     import tmg
     from tmg import serve

@@ -154,6 +154,7 @@ struct tmg_solver {
     bool ebe_lds = true;              // SRC_X applies of the spectral form with the node plane staged through LDS (k_ebe_lds)
     bool mr_lds = true;               // FP64 multiresolution fine level through k_ebe_lds as well (FP32: no gain)
     bool fin_consumer = true;         // rho/beta and alpha reduced in their consumers' blocks (no k_finalize<1,2>)
+    bool debug_fail_first = false;    // TMG_DEBUG_FAIL_FIRST_FACTOR=1: the first coarse factorization reports failure
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
     KWht<float> kw32{};
@@ -1315,6 +1316,14 @@ int refresh(tmg_solver *h) {
     // a nested dissection, or the dense operator, Cholesky and inverse
     DevLevel &lc = h->lev[h->L - 1];
     const int64_t nc = h->nc;
+    if (h->debug_fail_first) {
+        // test hook for the callers' failure handling: the first build of
+        // this handle reports a failed factorization (all kernels ran)
+        h->debug_fail_first = false;
+        HIPX(hipStreamSynchronize(h->st));
+        h->err = "coarse factorization failure forced for testing (TMG_DEBUG_FAIL_FIRST_FACTOR)";
+        return TMG_ERR_NOT_SPD;
+    }
     if (h->coarse_nd) {
         TRY(nd_factor(h));
         HIPX(hipStreamSynchronize(h->st));
@@ -1501,6 +1510,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_MR_LDS")) h->mr_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_FIN_CONSUMER")) h->fin_consumer = ev[0] != '0';
+    h->debug_fail_first = getenv_flag("TMG_DEBUG_FAIL_FIRST_FACTOR");
     if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
     if (const char *ev = getenv("TMG_STALL_FACTOR")) h->stall_factor = atof(ev);
