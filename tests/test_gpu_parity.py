@@ -515,6 +515,24 @@ def test_nested_dissection_coarse_solve_matches_dense_and_oracle(oracle_mod, mon
     _check_solution(oracle_mod, o, prob, u, it, it_or, u_or, 1e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_consumer_side_pcg_reductions_are_bitwise_the_finalize_kernels(monkeypatch, precision):
+    """rho/beta and alpha reduced by every block of k_dir_xupdate and
+    k_r_update (the default) give bitwise the iterates of the one-block
+    k_finalize launches (TMG_FIN_CONSUMER=0): same reduction order."""
+    prob = P.cantilever(32, 16, 16)
+    x = P.random_density(32, 16, 16, 41)
+    out = {}
+    for env in ("0", "1"):
+        monkeypatch.setenv("TMG_FIN_CONSUMER", env)
+        s = _setup(prob, x, 4, precision)
+        out[env] = s.solve(prob.f)
+        s.close()
+    (u0, it0, rr0, rc0), (u1, it1, rr1, rc1) = out["0"], out["1"]
+    assert rc0 == 0 and rc1 == 0 and it0 == it1 and rr0 == rr1
+    assert np.array_equal(u0, u1)
+
+
 @pytest.mark.parametrize("leaf,cycle", [(8, 0), (27, 0), (8, 1), (27, 2)])
 def test_nested_dissection_with_roller_supports(oracle_mod, monkeypatch, leaf, cycle):
     """Nested dissection on a quarter-symmetry beam whose supports fix single
