@@ -533,6 +533,29 @@ def test_consumer_side_pcg_reductions_are_bitwise_the_finalize_kernels(monkeypat
     assert np.array_equal(u0, u1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_prolongation_per_coarse_node_is_bitwise_the_per_fine_node_form(monkeypatch, precision):
+    """k_prolong_add4 (one thread per coarse node and fine plane, the default)
+    gives bitwise the values of k_prolong_add (one thread per fine node,
+    TMG_PROLONG4=0): same terms, weights and order. Checked on every level's
+    prolongation and on whole solves (W-cycle: each level prolongs twice)."""
+    prob = P.cantilever(32, 16, 16)
+    x = P.random_density(32, 16, 16, 43)
+    out = {}
+    for env in ("0", "1"):
+        monkeypatch.setenv("TMG_PROLONG4", env)
+        s = _setup(prob, x, 4, precision, cycle=1)
+        pro = [s.prolong(lev, np.random.default_rng(lev).standard_normal(s.nfree_level(lev + 1)))
+               for lev in range(1, 4)]
+        out[env] = (pro, s.solve(prob.f))
+        s.close()
+    (p0, (u0, it0, rr0, rc0)), (p1, (u1, it1, rr1, rc1)) = out["0"], out["1"]
+    for a, b in zip(p0, p1):
+        assert np.array_equal(a, b)
+    assert rc0 == 0 and rc1 == 0 and it0 == it1 and rr0 == rr1
+    assert np.array_equal(u0, u1)
+
+
 @pytest.mark.parametrize("leaf,cycle", [(8, 0), (27, 0), (8, 1), (27, 2)])
 def test_nested_dissection_with_roller_supports(oracle_mod, monkeypatch, leaf, cycle):
     """Nested dissection on a quarter-symmetry beam whose supports fix single

@@ -1520,6 +1520,84 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(int fnx, int fny, int fnz, 
     xn[0] = xv[0]; xn[1] = xv[1]; xn[2] = xv[2];
 }
 
+// The same x_f (+)= P e_c with one thread per coarse in-plane node (X, Y) and
+// fine plane iz (blockIdx.y): the thread loads the 2 x 2 x 2 coarse nodes
+// around its fine nodes (2X + ox, 2Y + oy), ox, oy in {0, 1}, once and writes
+// those up to four fine nodes. Each fine value is the same sum, in the same
+// order and with the same weights, as in the one-thread-per-fine-node form
+// above (bitwise equal results), from a quarter of the coarse loads.
+// Requires fnx = 2 cnx - 1, fny = 2 cny - 1, fnz = 2 cnz - 1.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_prolong_add4(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
+                                                      const T *__restrict__ ec, const uint8_t *__restrict__ fmask,
+                                                      T *__restrict__ xf, int add, const int *done) {
+    if (done && *done) return;
+    const int p = (int)blockIdx.x * TPB + (int)threadIdx.x;
+    if (p >= cnx * cny) return;
+    const int iz = (int)blockIdx.y, X = p / cny, Y = p - X * cny;
+    const int64_t cplane = (int64_t)cnx * cny;
+    const int zl = iz >> 1, zh = (iz + 1) >> 1;
+    const int X1 = min(X + 1, cnx - 1), Y1 = min(Y + 1, cny - 1);
+    const T wz[2] = {(iz & 1) ? T(0.5) : T(1), (iz & 1) ? T(0.5) : T(0)};
+    // e[a][b][d]: coarse node (z_a, X + b, Y + d), clamped at the last node
+    T e[2][2][2][3];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int64_t J = (int64_t)(a ? zh : zl) * cplane + (int64_t)(b ? X1 : X) * cny + (d ? Y1 : Y);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) e[a][b][d][c] = ec[3 * J + c];
+            }
+    // the fine operands of all four nodes are loaded before any is used (at
+    // clamped, in-range nodes: the last coarse node of a row has one fine
+    // node in that direction), so their loads are in flight together
+    const int64_t fplane = (int64_t)fnx * fny;
+    int64_t fi[2][2];
+    T xo[2][2][3];
+    uint8_t mk[2][2][3];
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const int ix = min(2 * X + ox, fnx - 1), iy = min(2 * Y + oy, fny - 1);
+            fi[ox][oy] = (int64_t)iz * fplane + (int64_t)ix * fny + iy;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                xo[ox][oy][c] = xf[3 * fi[ox][oy] + c];
+                mk[ox][oy][c] = fmask[3 * fi[ox][oy] + c];
+            }
+        }
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox) {
+        // an even fine index has its co-located coarse node twice (weights
+        // 1 and 0), an odd one its two neighbours (1/2 each)
+        const T wx[2] = {ox ? T(0.5) : T(1), ox ? T(0.5) : T(0)};
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const T wy[2] = {oy ? T(0.5) : T(1), oy ? T(0.5) : T(0)};
+            T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        const T w = wz[a] * wx[b] * wy[d];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) acc[c] += w * e[a][ox ? b : 0][oy ? d : 0][c];
+                    }
+            if (2 * X + ox < fnx && 2 * Y + oy < fny) {
+                T *xn = xf + 3 * fi[ox][oy];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xn[c] = (add ? xo[ox][oy][c] : T(0)) + (mk[ox][oy][c] ? acc[c] : T(0));
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Vector kernels.
 // ---------------------------------------------------------------------------

@@ -154,6 +154,7 @@ struct tmg_solver {
     bool ebe_lds = true;              // SRC_X applies of the spectral form with the node plane staged through LDS (k_ebe_lds)
     bool mr_lds = true;               // FP64 multiresolution fine level through k_ebe_lds as well (FP32: no gain)
     bool fin_consumer = true;         // rho/beta and alpha reduced in their consumers' blocks (no k_finalize<1,2>)
+    bool prolong4 = true;             // prolongation with one thread per coarse node and fine plane (k_prolong_add4)
     bool debug_fail_first = false;    // TMG_DEBUG_FAIL_FIRST_FACTOR=1: the first coarse factorization reports failure
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
@@ -566,6 +567,18 @@ void launch_stencil(tmg_solver *h, int l, const StArgs<T> &a) {
     hipLaunchKernelGGL((k_stencil<T, SRC, EPI>), dim3(nblk(lv.nn)), dim3(TPB), 0, h->st, ST_ARGS(a));
 }
 
+// x_l (+)= P e_{l+1} on level l's free DOFs (Implementation/Prolongation.m)
+template <typename T>
+void prolong_add(tmg_solver *h, int l, const T *ec, T *x, int add, const int *done) {
+    const DevLevel &lf = h->lev[l], &lc = h->lev[l + 1];
+    if (h->prolong4)
+        hipLaunchKernelGGL((k_prolong_add4<T>), plane_grid(lc.nx, lc.ny, lf.nz), dim3(TPB), 0, h->st, lf.nx, lf.ny,
+                           lf.nz, lc.nx, lc.ny, lc.nz, ec, lf.mask, x, add, done);
+    else
+        hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(lf.nx, lf.ny, lf.nz), dim3(TPB), 0, h->st, lf.nx, lf.ny,
+                           lf.nz, lc.nx, lc.ny, lc.nz, ec, lf.mask, x, add, done);
+}
+
 template <typename T>
 const T *E1of(tmg_solver *h);
 template <> const double *E1of<double>(tmg_solver *h) { return h->E1_64; }
@@ -731,8 +744,7 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
     hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st,    // rH = P' rh
                        lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, xb, lc.mask, (T *)lc.b, done);
     T *ec = coarse_correction<T>(h, l, done);
-    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(lv.nx, lv.ny, lv.nz), dim3(TPB), 0, h->st,  // x += P eH
-                       lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, (const T *)ec, lv.mask, xa, 1, done);
+    prolong_add<T>(h, l, (const T *)ec, xa, 1, done);                            // x += P eH
     for (int s = 0; s < nu2; ++s) {
         apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
         std::swap(xa, xb);
@@ -785,8 +797,7 @@ T *fmv(tmg_solver *h, int l, const T *b, const int *done) {
     }
     T *xa = (T *)lv.x;
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, T(0), xa);
-    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(lv.nx, lv.ny, lv.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny,
-                       lv.nz, lc.nx, lc.ny, lc.nz, (const T *)xc, lv.mask, xa, 1, done);   // x = P x
+    prolong_add<T>(h, l, (const T *)xc, xa, 1, done);                                  // x = P x
     return vcycle<T>(h, l, b, done, xa);                                                // VCycle(A, b, x, ...)
 }
 
@@ -820,8 +831,7 @@ T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done, bool x0_ready 
     hipLaunchKernelGGL((k_restrict<T>), plane_grid(l1.nx, l1.ny, l1.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
                        l1.nz, (const T *)xb, l1.mask, (T *)l1.b, done);
     T *ec = coarse_correction<T>(h, 0, done);
-    hipLaunchKernelGGL((k_prolong_add<T>), plane_grid(l0.nx, l0.ny, l0.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
-                       l1.nz, (const T *)ec, l0.mask, xa, 1, done);
+    prolong_add<T>(h, 0, (const T *)ec, xa, 1, done);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.x = xa; b.r = r; b.out = xb; b.partial = h->partial; b.done = done;
     if (h->fin_fused) { b.sc = h->d_sc; b.fin = FIN_RHO; }       // rho = r's, beta: in the last block
@@ -1510,6 +1520,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_EBE_LDS")) h->ebe_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_MR_LDS")) h->mr_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_FIN_CONSUMER")) h->fin_consumer = ev[0] != '0';
+    if (const char *ev = getenv("TMG_PROLONG4")) h->prolong4 = ev[0] != '0';
     h->debug_fail_first = getenv_flag("TMG_DEBUG_FAIL_FIRST_FACTOR");
     if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
@@ -1993,8 +2004,7 @@ int tmg_prolong(tmg_solver *h, int level, const double *xc, double *xf) {
     TRY(dalloc(h, &df, lf.ndof));
     int rc = upload_free(h, level, xc, dc);
     if (!rc) {
-        hipLaunchKernelGGL((k_prolong_add<double>), plane_grid(lf.nx, lf.ny, lf.nz), dim3(TPB), 0, h->st, lf.nx, lf.ny, lf.nz, lc.nx,
-                           lc.ny, lc.nz, dc, lf.mask, df, 0, nullptr);
+        prolong_add<double>(h, level - 1, dc, df, 0, nullptr);
         rc = check_launch(h);
     }
     if (!rc) rc = download_free(h, level - 1, df, xf);
