@@ -15,8 +15,10 @@
 // of the operator's entries and of the children's update matrices, then per
 // front rocSOLVER/rocBLAS potrf, trsm, syrk (the update matrix), trtri and
 // trmm, which leave M = [L_PP^-1 ; L_BP L_PP^-1] in the front's first p
-// columns. The fronts of one height are independent and go round-robin to
-// up to 4 side streams.
+// columns. All of it runs in order on the handle's stream: with the fronts
+// of one height spread over concurrent side streams (each with its own
+// rocBLAS handle), identical set-ups gave different factors from run to run,
+// some of them not positive definite.
 // The solve is two sweeps of matrix-vector products over the fronts of one
 // height per launch (k_nd_fwd, k_nd_bwd).
 
@@ -278,39 +280,10 @@ int nd_upload(tmg_solver *h, NdHost &o) {
         h->err = "upload of the coarse nested-dissection structure failed";
         return TMG_ERR_HIP;
     }
-    // side streams for the factorization (not for handles that share a
-    // caller's stream and rocBLAS handle, such as the server's)
-    int ns = 4;
-    if (const char *ev = getenv("TMG_ND_STREAMS")) ns = std::max(1, std::min(atoi(ev), 4));
-    if (h->own_stream && h->own_rb && ns > 1) {
-        for (int k = 0; k < 5; ++k)
-            if (hipEventCreateWithFlags(&h->nd_ev[k], hipEventDisableTiming) != hipSuccess) ns = 1;
-        for (int k = 0; k < ns; ++k) {
-            if (hipStreamCreateWithFlags(&h->nd_st[k], hipStreamNonBlocking) != hipSuccess ||
-                rocblas_create_handle(&h->nd_rb[k]) != rocblas_status_success) {
-                h->err = "side stream or rocBLAS handle creation failed";
-                return TMG_ERR_HIP;
-            }
-            rocblas_set_stream(h->nd_rb[k], h->nd_st[k]);
-        }
-        h->nd_ns = ns;
-    }
     return 0;
 }
 
 void nd_free(tmg_solver *h) {
-    for (int k = 0; k < 4; ++k) {
-        if (h->nd_st[k]) (void)hipStreamSynchronize(h->nd_st[k]);
-        if (h->nd_rb[k]) rocblas_destroy_handle(h->nd_rb[k]);
-        if (h->nd_st[k]) (void)hipStreamDestroy(h->nd_st[k]);
-        h->nd_rb[k] = nullptr;
-        h->nd_st[k] = nullptr;
-    }
-    for (int k = 0; k < 5; ++k) {
-        if (h->nd_ev[k]) (void)hipEventDestroy(h->nd_ev[k]);
-        h->nd_ev[k] = nullptr;
-    }
-    h->nd_ns = 0;
     dfree(h, h->nd_xc); dfree(h, h->nd_y); dfree(h, h->nd_T); dfree(h, h->nd_tp); dfree(h, h->nd_Mt); dfree(h, h->nd_F);
     dfree(h, h->nd_bw);
     dfree(h, h->nd_fw); dfree(h, h->nd_ea); dfree(h, h->nd_info); dfree(h, h->nd_bsrc); dfree(h, h->nd_tsrc);
@@ -329,24 +302,6 @@ int nd_factor(tmg_solver *h) {
     TRY(check_launch(h));
     rocblas_set_stream(h->rb, h->st);
     const double one = 1.0, mone = -1.0;
-    // fronts of one height go round-robin to the side streams, which wait for
-    // the height's assembly on the handle's stream and are joined back to it
-    const int ns = std::max(h->nd_ns, 1);
-    auto fork = [&]() -> int {
-        if (ns < 2) return 0;
-        HIPX(hipEventRecord(h->nd_ev[0], h->st));
-        for (int k = 0; k < ns; ++k) HIPX(hipStreamWaitEvent(h->nd_st[k], h->nd_ev[0], 0));
-        return 0;
-    };
-    auto join = [&]() -> int {
-        if (ns < 2) return 0;
-        for (int k = 0; k < ns; ++k) {
-            HIPX(hipEventRecord(h->nd_ev[k + 1], h->nd_st[k]));
-            HIPX(hipStreamWaitEvent(h->st, h->nd_ev[k + 1], 0));
-        }
-        return 0;
-    };
-    auto handle_of = [&](int i) { return ns < 2 ? h->rb : h->nd_rb[i % ns]; };
     for (int hh = 0; hh < h->nd_nheight; ++hh) {
         const int e0 = h->nd_ea_start[hh], e1 = h->nd_ea_start[hh + 1];
         if (e1 > e0)
@@ -354,11 +309,10 @@ int nd_factor(tmg_solver *h) {
                                (const int2 *)h->nd_ea + e0, (const NdFront *)h->nd_fronts, (const int *)h->nd_cmap,
                                h->nd_F);
         TRY(check_launch(h));
-        TRY(fork());
         for (int f = h->nd_hstart[hh]; f < h->nd_hstart[hh + 1]; ++f) {
             const NdFront &F = h->nd_fr[f];
             if (F.p == 0) continue;
-            rocblas_handle rb = handle_of(f - h->nd_hstart[hh]);
+            rocblas_handle rb = h->rb;
             double *Fp = h->nd_F + F.F_off;
             double *FB = Fp + F.p;                              // rows B, columns P
             double *FBB = Fp + F.p + (int64_t)F.p * F.ld;       // rows B, columns B
@@ -371,13 +325,11 @@ int nd_factor(tmg_solver *h) {
                                        FBB, F.ld);
             }
             if (st != rocblas_status_success) {
-                (void)join();
                 h->err = "coarse multifrontal factorization: rocBLAS/rocSOLVER call failed (status " +
                          std::to_string((int)st) + ")";
                 return TMG_ERR_HIP;
             }
         }
-        TRY(join());
     }
     // pivot failures before the inverses of the diagonal factors
     std::vector<int> info(h->nd_nfront, 0);
@@ -389,11 +341,10 @@ int nd_factor(tmg_solver *h) {
                      std::to_string(info[f]) + ")";
             return TMG_ERR_NOT_SPD;
         }
-    TRY(fork());
     for (int f = 0; f < h->nd_nfront; ++f) {
         const NdFront &F = h->nd_fr[f];
         if (F.p == 0) continue;
-        rocblas_handle rb = handle_of(f);
+        rocblas_handle rb = h->rb;
         double *Fp = h->nd_F + F.F_off;
         rocblas_status st = rocsolver_dtrtri(rb, rocblas_fill_lower, rocblas_diagonal_non_unit, F.p, Fp, F.ld,
                                              h->nd_info + f);
@@ -401,13 +352,11 @@ int nd_factor(tmg_solver *h) {
             st = rocblas_dtrmm(rb, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
                                rocblas_diagonal_non_unit, F.b, F.p, &one, Fp, F.ld, Fp + F.p, F.ld, Fp + F.p, F.ld);
         if (st != rocblas_status_success) {
-            (void)join();
             h->err = "coarse multifrontal factorization: rocBLAS/rocSOLVER call failed (status " +
                      std::to_string((int)st) + ")";
             return TMG_ERR_HIP;
         }
     }
-    TRY(join());
     if (h->nd_ntp > 0)
         hipLaunchKernelGGL(k_nd_transpose, dim3((unsigned)h->nd_ntp), dim3(TPB), 0, h->st, (const int2 *)h->nd_tp,
                            (const NdFront *)h->nd_fronts, (const double *)h->nd_F, h->nd_Mt);

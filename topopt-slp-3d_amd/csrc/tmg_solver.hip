@@ -107,10 +107,6 @@ struct tmg_solver {
     int64_t nd_nent = 0, nd_Fsize = 0, nd_ntp = 0;
     // the fronts of one height are factorised on up to 4 side streams, each
     // with its own rocBLAS handle (handles that own their stream only)
-    int nd_ns = 0;
-    hipStream_t nd_st[4] = {nullptr, nullptr, nullptr, nullptr};
-    rocblas_handle nd_rb[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t nd_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     rocblas_handle rb = nullptr;
     // PCG
     double *px = nullptr, *pr = nullptr, *pd = nullptr, *pd2 = nullptr, *pq = nullptr, *pf = nullptr;
