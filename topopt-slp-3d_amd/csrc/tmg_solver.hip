@@ -1280,6 +1280,11 @@ void element_stiffness(double el, double eh, double ew, double nu, double *k) {
     element_stiffness_box(el, eh, ew, nu, lo, 2.0, k);
 }
 
+std::mutex &factor_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 int refresh(tmg_solver *h) {
     DevLevel &l0 = h->lev[0];
     const double om = h->mg.omega;
@@ -1330,6 +1335,13 @@ int refresh(tmg_solver *h) {
         h->err = "coarse factorization failure forced for testing (TMG_DEBUG_FAIL_FIRST_FACTOR)";
         return TMG_ERR_NOT_SPD;
     }
+    // One coarse factorization at a time on the device, process-wide: the
+    // rocSOLVER/rocBLAS calls of handles that set up concurrently (the
+    // server's slots) never overlap. Held from the enqueue to the stream's
+    // completion; the level operators above are finished first, so the wait
+    // of another handle covers factorization work only.
+    HIPX(hipStreamSynchronize(h->st));
+    std::lock_guard<std::mutex> factor_lock(factor_mutex());
     if (h->coarse_nd) {
         TRY(nd_factor(h));
         HIPX(hipStreamSynchronize(h->st));
