@@ -58,6 +58,8 @@ def parse():
                     help="profiling aid: only launch this solver kernel (tmg_time_kernel id) --kernel-reps times "
                          "on the workload and print its average; no JSON bench line")
     ap.add_argument("--probe-precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--probe-mr", type=int, default=1, choices=[1, 2],
+                    help="probe on the multiresolution problem (mr.n = 2, W-cycle, uniform density 0.3) of --grid")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: only run three elementwise kernels (scale, copy, add) on vectors of this "
                          "many bytes (calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE)")
@@ -162,7 +164,7 @@ def iteration_bytes(prob, L, prec):
 def probe(args):
     """Profiling aid: launch one solver kernel (or a copy of known size)
     --kernel-reps times; prints its HIP-event average. Kernel ids are those of
-    tmg_time_kernel (0 FP64 fine apply q = K d, 1 fine Jacobi sweep, 2 level-2
+    tmg_time_kernel (0 FP64 fine apply q = K d, 1 fine Jacobi sweep, 2 second-level
     sweep, 3 V-cycle, 4/5 fused fine residual / post-sweep, 6 fused direction
     update + apply, 7/8 fine residual / post-sweep on the FP64 PCG residual)."""
     import torch
@@ -189,12 +191,17 @@ def probe(args):
     from tmg import problem as P
     nelx, nely, nelz = (int(v) for v in args.grid.split("x"))
     prob = P.cantilever(nelx, nely, nelz)
-    x = P.random_density(nelx, nely, nelz, args.seed)
-    s = tmg.Solver(prob, args.levels, precision=args.probe_precision)
+    if args.probe_mr == 2:
+        import numpy as np
+        x = np.full(8 * prob.nelem, 0.3)
+        s = tmg.Solver(prob, args.levels, precision=args.probe_precision, mr=2, cycle=1)
+    else:
+        x = P.random_density(nelx, nely, nelz, args.seed)
+        s = tmg.Solver(prob, args.levels, precision=args.probe_precision)
     s.update_density(x)
     ms = s.time_kernel(args.probe_kernel, args.kernel_reps)
-    log(json.dumps({"kernel": args.probe_kernel, "precision": args.probe_precision, "avg_ms": ms,
-                    "reps": args.kernel_reps, "n_dof": prob.ndof, "n_el": prob.nelem}))
+    log(json.dumps({"kernel": args.probe_kernel, "precision": args.probe_precision, "mr": args.probe_mr,
+                    "avg_ms": ms, "reps": args.kernel_reps, "n_dof": prob.ndof, "n_el": prob.nelem}))
 
 
 def slp_workload(args):
