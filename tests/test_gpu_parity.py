@@ -534,6 +534,30 @@ def test_consumer_side_pcg_reductions_are_bitwise_the_finalize_kernels(monkeypat
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("cycle", [0, 1])
+def test_restriction_with_first_sweep_is_bitwise_the_separate_sweep(monkeypatch, precision, cycle):
+    """The restriction that also forms the next level's first Jacobi sweep
+    from zero (the default) gives bitwise the cycle and the solve of the
+    separate k_jacobi_zero launch (TMG_RESTRICT_JZ=0). The small levels'
+    fused sweeps are switched off (TMG_FUSED_MAXNODES=0) so that every
+    level between the finest and the coarsest takes that path."""
+    prob = P.cantilever(32, 16, 16)
+    x = P.random_density(32, 16, 16, 47)
+    b = np.random.default_rng(47).standard_normal(prob.nfree)
+    monkeypatch.setenv("TMG_FUSED_MAXNODES", "0")
+    out = {}
+    for env in ("0", "1"):
+        monkeypatch.setenv("TMG_RESTRICT_JZ", env)
+        s = _setup(prob, x, 4, precision, cycle=cycle)
+        out[env] = (s.vcycle(b), s.solve(prob.f))
+        s.close()
+    (v0, (u0, it0, rr0, rc0)), (v1, (u1, it1, rr1, rc1)) = out["0"], out["1"]
+    assert np.array_equal(v0, v1)
+    assert rc0 == 0 and rc1 == 0 and it0 == it1 and rr0 == rr1
+    assert np.array_equal(u0, u1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_prolongation_per_coarse_node_is_bitwise_the_per_fine_node_form(monkeypatch, precision):
     """k_prolong_add4 (one thread per coarse node and fine plane, the default)
     gives bitwise the values of k_prolong_add (one thread per fine node,

@@ -1430,10 +1430,13 @@ __global__ __launch_bounds__(ST3_THREADS) void k_stencil3(int nx_, int ny_, int 
 // blockIdx.x * TPB + threadIdx.x = in-plane index x * ny + y (32-bit).
 // Index arithmetic is per thread, once; boundary neighbours are clamped and
 // given weight zero, so there are no branches.
-template <typename T>
+// JZ: also the coarse level's first Jacobi sweep from zero, xc = invc * bc
+// (the product k_jacobi_zero forms, on the value just stored).
+template <typename T, bool JZ = false>
 __global__ __launch_bounds__(TPB) void k_restrict(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
                                                   const T *__restrict__ rf, const uint8_t *__restrict__ cmask,
-                                                  T *__restrict__ bc, const int *done) {
+                                                  T *__restrict__ bc, const int *done, const T *__restrict__ invc = nullptr,
+                                                  T *__restrict__ xc = nullptr) {
     if (done && *done) return;
     const int p = (int)blockIdx.x * TPB + (int)threadIdx.x;
     if (p >= cnx * cny) return;
@@ -1470,7 +1473,11 @@ __global__ __launch_bounds__(TPB) void k_restrict(int fnx, int fny, int fnz, int
     }
     const int64_t I = (int64_t)Iz * cnx * cny + p;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) bc[3 * I + c] = cmask[3 * I + c] ? acc[c] : T(0);
+    for (int c = 0; c < 3; ++c) {
+        const T v = cmask[3 * I + c] ? acc[c] : T(0);
+        bc[3 * I + c] = v;
+        if (JZ) xc[3 * I + c] = invc[3 * I + c] * v;
+    }
 }
 
 // x_f (+)= P e_c on free fine DOFs: each fine node takes its co-located or

@@ -151,6 +151,7 @@ struct tmg_solver {
     bool mr_lds = true;               // FP64 multiresolution fine level through k_ebe_lds as well (FP32: no gain)
     bool fin_consumer = true;         // rho/beta and alpha reduced in their consumers' blocks (no k_finalize<1,2>)
     bool prolong4 = true;             // prolongation with one thread per coarse node and fine plane (k_prolong_add4)
+    bool restrict_jz = true;          // the restriction also forms the next level's first sweep from zero
     bool debug_fail_first = false;    // TMG_DEBUG_FAIL_FIRST_FACTOR=1: the first coarse factorization reports failure
     int xcd_wht = 1;                  // XCD-contiguous tile order for the spectral apply
     KWht<double> kw64{};
@@ -677,19 +678,40 @@ void coarse_solve(tmg_solver *h, const T *b, T *x, const int *done) {
 // One V(nu1,nu2) cycle from a zero initial guess at level l
 // (Implementation/VCycle.m:23-43). Returns the buffer holding the result.
 template <typename T>
-T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0 = nullptr);
+T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0 = nullptr, bool jz = false);
+
+// Level m's cycle from zero takes the unfused path with a first sweep from
+// zero (x = M \ b): the restriction into level m can form that sweep.
+bool restrict_jz_ok(const tmg_solver *h, int m) {
+    if (!h->restrict_jz || m <= 0 || m >= h->L - 1 || h->mg.smoothiter1 < 1) return false;
+    const bool fused = h->mg.smoothiter1 == 1 && h->mg.smoothiter2 == 1 && h->fused_coarse &&
+                       h->lev[m].nn <= h->fused_maxnodes;
+    return !fused;
+}
+
+// rH = P' rh into level l + 1's b; with jz also x = M \ rH into its x.
+template <typename T>
+void restrict_to(tmg_solver *h, int l, const T *rf, bool jz, const int *done) {
+    const DevLevel &lv = h->lev[l], &lc = h->lev[l + 1];
+    if (jz)
+        hipLaunchKernelGGL((k_restrict<T, true>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny,
+                           lv.nz, lc.nx, lc.ny, lc.nz, rf, lc.mask, (T *)lc.b, done, invof<T>(lc), (T *)lc.x);
+    else
+        hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz,
+                           lc.nx, lc.ny, lc.nz, rf, lc.mask, (T *)lc.b, done, nullptr, nullptr);
+}
 
 // Coarse correction of a cycle at level l: the direct solve below the
 // coarsest-but-one level, else one cycle (V) or two, the second from the
 // first's result (W, Implementation/WCycle.m:18-20).
 template <typename T>
-T *coarse_correction(tmg_solver *h, int l, const int *done) {
+T *coarse_correction(tmg_solver *h, int l, const int *done, bool jz = false) {
     DevLevel &lc = h->lev[l + 1];
     if (l + 1 == h->L - 1) {
         coarse_solve<T>(h, (const T *)lc.b, (T *)lc.x, done);                      // eH = R \ (R' \ rH)
         return (T *)lc.x;
     }
-    T *e = vcycle<T>(h, l + 1, (const T *)lc.b, done);
+    T *e = vcycle<T>(h, l + 1, (const T *)lc.b, done, nullptr, jz);
     if (h->mg.cycle == 1) e = vcycle<T>(h, l + 1, (const T *)lc.b, done, e);
     return e;
 }
@@ -699,7 +721,7 @@ T *coarse_correction(tmg_solver *h, int l, const int *done) {
 // level's two work vectors, or nullptr for zero. Returns the buffer holding
 // the result (one of the level's two work vectors).
 template <typename T>
-T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
+T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0, bool jz) {
     DevLevel &lv = h->lev[l];
     DevLevel &lc = h->lev[l + 1];
     T *xa = (T *)lv.x, *xb = (T *)lv.t;
@@ -710,9 +732,9 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
         StArgs<T> a = st_args<T>(h, l);
         a.b = b; a.out = xb; a.done = done;
         launch_stencil<T, SRC_JZ, EPI_RESID>(h, l, a);
-        hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st, lv.nx, lv.ny, lv.nz, lc.nx, lc.ny,
-                           lc.nz, xb, lc.mask, (T *)lc.b, done);
-        T *ec = coarse_correction<T>(h, l, done);
+        const bool jzc = restrict_jz_ok(h, l + 1);
+        restrict_to<T>(h, l, xb, jzc, done);
+        T *ec = coarse_correction<T>(h, l, done, jzc);
         StArgs<T> c = st_args<T>(h, l);
         c.b = b; c.ec = ec; c.out = xa; c.done = done;
         launch_stencil<T, SRC_POST, EPI_JACOBI>(h, l, c);
@@ -727,8 +749,11 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
             std::swap(xa, xb);
         }
     } else if (nu1 >= 1) {
-        // first sweep from x0 = 0: x = M \ b (Implementation/Smooth.m:17 with x = 0)
-        hipLaunchKernelGGL((k_jacobi_zero<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, b, invof<T>(lv), xa, done);
+        // first sweep from x0 = 0: x = M \ b (Implementation/Smooth.m:17 with x = 0);
+        // with jz the restriction that produced b wrote it
+        if (!jz)
+            hipLaunchKernelGGL((k_jacobi_zero<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, b, invof<T>(lv), xa,
+                               done);
         for (int s = 1; s < nu1; ++s) {
             apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
             std::swap(xa, xb);
@@ -737,9 +762,9 @@ T *vcycle(tmg_solver *h, int l, const T *b, const int *done, T *x0) {
         hipLaunchKernelGGL((k_fill<T>), dim3(nblk(lv.ndof)), dim3(TPB), 0, h->st, lv.ndof, T(0), xa);
     }
     apply<T, EPI_RESID>(h, l, xa, b, xb, nullptr, done);                          // rh = b - A x
-    hipLaunchKernelGGL((k_restrict<T>), plane_grid(lc.nx, lc.ny, lc.nz), dim3(TPB), 0, h->st,    // rH = P' rh
-                       lv.nx, lv.ny, lv.nz, lc.nx, lc.ny, lc.nz, xb, lc.mask, (T *)lc.b, done);
-    T *ec = coarse_correction<T>(h, l, done);
+    const bool jzc = restrict_jz_ok(h, l + 1);
+    restrict_to<T>(h, l, xb, jzc, done);                                           // rH = P' rh
+    T *ec = coarse_correction<T>(h, l, done, jzc);
     prolong_add<T>(h, l, (const T *)ec, xa, 1, done);                            // x += P eH
     for (int s = 0; s < nu2; ++s) {
         apply<T, EPI_JACOBI>(h, l, xa, b, xb, nullptr, done);
@@ -762,9 +787,9 @@ T *vcycle_fine_fused(tmg_solver *h, const double *r, const int *done) {
     EbeArgs<T, T> a = ebe_args<T, T>(h);
     a.r = r; a.done = done; a.out = (T *)l0.t;
     launch_ebe<T, T, SRC_JZ, EPI_RESIDR>(h, a);
-    hipLaunchKernelGGL((k_restrict<T>), plane_grid(l1.nx, l1.ny, l1.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
-                       l1.nz, (const T *)l0.t, l1.mask, (T *)l1.b, done);
-    T *ec = coarse_correction<T>(h, 0, done);
+    const bool jzc = restrict_jz_ok(h, 1);
+    restrict_to<T>(h, 0, (const T *)l0.t, jzc, done);
+    T *ec = coarse_correction<T>(h, 0, done, jzc);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.r = r; b.ec = ec; b.done = done; b.out = (T *)l0.x; b.partial = h->partial;
     launch_ebe<T, T, SRC_POST, EPI_POSTR>(h, b);
@@ -824,9 +849,9 @@ T *vcycle_fine_r(tmg_solver *h, const double *r, const int *done, bool x0_ready 
     EbeArgs<T, T> a = ebe_args<T, T>(h);
     a.x = xa; a.r = r; a.out = xb; a.done = done;
     launch_ebe<T, T, SRC_X, EPI_RESIDR>(h, a);
-    hipLaunchKernelGGL((k_restrict<T>), plane_grid(l1.nx, l1.ny, l1.nz), dim3(TPB), 0, h->st, l0.nx, l0.ny, l0.nz, l1.nx, l1.ny,
-                       l1.nz, (const T *)xb, l1.mask, (T *)l1.b, done);
-    T *ec = coarse_correction<T>(h, 0, done);
+    const bool jzc = restrict_jz_ok(h, 1);
+    restrict_to<T>(h, 0, (const T *)xb, jzc, done);
+    T *ec = coarse_correction<T>(h, 0, done, jzc);
     prolong_add<T>(h, 0, (const T *)ec, xa, 1, done);
     EbeArgs<T, T> b = ebe_args<T, T>(h);
     b.x = xa; b.r = r; b.out = xb; b.partial = h->partial; b.done = done;
@@ -1529,6 +1554,7 @@ int setup_impl(const tmg_problem *prob, const tmg_mg_params *mg, int precision, 
     if (const char *ev = getenv("TMG_MR_LDS")) h->mr_lds = ev[0] != '0';
     if (const char *ev = getenv("TMG_FIN_CONSUMER")) h->fin_consumer = ev[0] != '0';
     if (const char *ev = getenv("TMG_PROLONG4")) h->prolong4 = ev[0] != '0';
+    if (const char *ev = getenv("TMG_RESTRICT_JZ")) h->restrict_jz = ev[0] != '0';
     h->debug_fail_first = getenv_flag("TMG_DEBUG_FAIL_FIRST_FACTOR");
     if (const char *ev = getenv("TMG_XDEFER")) h->xdefer = ev[0] != '0';
     if (const char *ev = getenv("TMG_STALL_WINDOW")) h->stall_window = std::max(1, atoi(ev));
